@@ -1,0 +1,139 @@
+/*
+ * phonofdtd — C ABI of the MI355X (gfx950) FDTD time-stepping library that
+ * replaces the step loop of phonomena's solver plugins.
+ *
+ * Every entry point returns int: 0 on success, a negative PF_ERR_* code on
+ * failure (never throws, never aborts).  pf_last_error() gives the message of
+ * the last failure on the calling thread.  A context is not thread-safe: one
+ * host thread drives it.  The library owns all device memory; callers own the
+ * host buffers they pass (plain pointers and element counts).
+ *
+ * Field arrays use the reference shapes and C order (z fastest), as in
+ * phonomena/simulation/grid.py:88-98:
+ *   ux (nx-1, ny, nz)   uy (nx, ny-1, nz)   uz (nx, ny, nz-1)
+ * where nx, ny, nz are node counts (cells + 1 per axis).
+ */
+#ifndef PHONOFDTD_H
+#define PHONOFDTD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PF_ABI_VERSION 1
+
+#define PF_OK 0
+#define PF_ERR_ARG (-1)     /* invalid argument or shape                 */
+#define PF_ERR_HIP (-2)     /* HIP runtime error (message has details)   */
+#define PF_ERR_STATE (-3)   /* call not valid in the current state       */
+#define PF_ERR_NOMEM (-4)   /* device or host allocation failed          */
+#define PF_ERR_RANGE (-5)   /* step count beyond the uploaded source     */
+
+/* Arithmetic of the sweep. */
+#define PF_FP64 0   /* FP64, the reference's expression order (parity mode)   */
+#define PF_FP32 1   /* FP32 fields, pre-folded coefficients (production)      */
+#define PF_FP32C 2  /* FP32 fields carried as hi+lo pairs: compensated
+                       (TwoSum/Kahan) time integration for long runs         */
+
+/* Sweep kernel. */
+#define PF_KERNEL_AUTO 0
+#define PF_KERNEL_NAIVE 1   /* one thread per node, stresses recomputed     */
+#define PF_KERNEL_TILED 2   /* LDS-staged (y,z) tiles, register march in x  */
+
+/* Field selectors for pf_get_field / pf_set_field. */
+#define PF_UX 0
+#define PF_UY 1
+#define PF_UZ 2
+#define PF_UX_OLD 3   /* previous level, as grid.ux_old after time_step */
+#define PF_UY_OLD 4
+#define PF_UZ_OLD 5
+
+typedef struct pf_ctx pf_ctx;
+typedef struct pf_batch pf_batch;
+
+/*
+ * Everything one mesh needs.  Replaces the state that
+ * BaseSolver.init (phonomena/simulation/base_solver.py:194-222) builds:
+ * the Grid spacings (grid.py:112-127), the Material arrays
+ * (material.py:48-63) and dt (material.py:80-93).
+ */
+typedef struct pf_desc {
+    int32_t nx, ny, nz;        /* node counts, each >= 4                       */
+    int32_t precision;         /* PF_FP64 | PF_FP32 | PF_FP32C                  */
+    int32_t kernel;            /* PF_KERNEL_*                                   */
+    int32_t n_materials;       /* 1..256                                        */
+    const double *fdx, *fdy, *fdz;   /* full spacings, nx-1 / ny-1 / nz-1       */
+    const double *sdx, *sdy, *sdz;   /* staggered spacings, nx-2 / ny-2 / nz-2  */
+    /* n_materials rows of 13: C00 C01 C02 C10 C11 C12 C20 C21 C22 (the normal
+       3x3 block of the 6x6 stiffness), C33 C44 C55 (shear diagonal), rho.   */
+    const double *materials;
+    const uint8_t *ids;        /* nx*ny*nz material index per node             */
+    double dt;
+    double dt2;                /* dt**2 exactly as the host computed it         */
+    const double *mur;         /* 12 Mur coefficients: faces x=max, y=0, y=max,
+                                  z=max (in that order), each (ux, uy, uz)      */
+    const double *sponge_x, *sponge_y, *sponge_z;  /* per-axis damping terms
+                                  (nx / ny / nz) or all NULL for none          */
+    const double *wave;        /* FP64 source table w(tt), tt = 0..wave_len-1  */
+    int64_t wave_len;          /* also the step budget of the context          */
+    int32_t probe[6];          /* surface-u_z probe grid: i0 si ni j0 sj nj
+                                  (ni == 0: no probes)                          */
+    int32_t record_every;      /* probe frame after every k-th step            */
+    int32_t probe_capacity;    /* ring size in frames                          */
+    int32_t tile_j;            /* tiled kernel: rows per tile (0 = default)    */
+    int32_t tile_x;            /* tiled kernel: x planes per tile (0 = all)    */
+} pf_desc;
+
+/* Library / device identification. */
+int pf_abi_version(void);
+const char *pf_last_error(void);
+int pf_device_info(char *name, int32_t cap, int32_t *cu_count, double *hbm_gib);
+
+/* Context life cycle: upload mesh, material, source; zero fields. */
+int pf_create(const pf_desc *desc, pf_ctx **out);
+int pf_destroy(pf_ctx *ctx);
+
+/*
+ * Advance nsteps time steps (the loop body of BaseSolver.run,
+ * base_solver.py:245-260: source, stresses, free surface, displacements,
+ * Mur faces, rotation).  Asynchronous on the context's stream.
+ */
+int pf_step(pf_ctx *ctx, int64_t nsteps);
+int pf_sync(pf_ctx *ctx);
+int pf_steps_done(pf_ctx *ctx, int64_t *out);
+
+/* Fields in reference shapes, converted to/from double. */
+int pf_get_field(pf_ctx *ctx, int32_t which, double *host, int64_t count);
+int pf_set_field(pf_ctx *ctx, int32_t which, const double *host, int64_t count);
+
+/* Probe frames recorded so far (oldest first, at most capacity). */
+int pf_get_probes(pf_ctx *ctx, double *host, int64_t cap_frames, int64_t *nframes);
+
+/* Energy audit of the newest level, summed in FP64 in a fixed order. */
+int pf_energy(pf_ctx *ctx, double *out);
+
+/*
+ * Timing: run nsteps steps with HIP events around every sweep and face
+ * launch on the context's stream; returns the mean duration of each (ms).
+ */
+int pf_profile(pf_ctx *ctx, int64_t nsteps, double *sweep_ms, double *face_ms);
+
+/*
+ * Batches: many contexts of one precision stepped together, one launch per
+ * stage over a combined tile work-list, the step loop captured in a HIP
+ * graph.  Each member stops at its own step budget (wave_len).
+ */
+int pf_batch_create(int32_t precision, pf_batch **out);
+int pf_batch_add(pf_batch *b, pf_ctx *ctx);
+int pf_batch_remove(pf_batch *b, pf_ctx *ctx);
+int pf_batch_step(pf_batch *b, int64_t nsteps);
+int pf_batch_sync(pf_batch *b);
+int pf_batch_set_tiles(pf_batch *b, int32_t tile_j, int32_t tile_x, int32_t use_graph);
+int pf_batch_destroy(pf_batch *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PHONOFDTD_H */

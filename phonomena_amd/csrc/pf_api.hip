@@ -1,0 +1,567 @@
+// C ABI of libphonofdtd (declared in include/phonofdtd.h): context life
+// cycle, stepping, field transfer, probes, energy audit, profiling.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/phonofdtd.h"
+#include "pf_energy.hpp"
+#include "pf_kernels.hpp"
+#include "pf_runtime.hpp"
+#include "pf_tiled.hpp"
+
+using namespace pf;
+
+// ------------------------------------------------------------ errors ----
+static thread_local char g_err[512] = "";
+
+int pf::fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+extern "C" const char* pf_last_error(void) { return g_err; }
+extern "C" int pf_abi_version(void) { return PF_ABI_VERSION; }
+
+extern "C" int pf_device_info(char* name, int32_t cap, int32_t* cu_count, double* hbm_gib) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    HIPCHK(hipGetDeviceProperties(&p, dev));
+    if (name && cap > 0) {
+        snprintf(name, (size_t)cap, "%s (%s)", p.name, p.gcnArchName);
+    }
+    if (cu_count) *cu_count = p.multiProcessorCount;
+    if (hbm_gib) *hbm_gib = (double)p.totalGlobalMem / (1024.0 * 1024.0 * 1024.0);
+    return PF_OK;
+}
+
+// ----------------------------------------------------------- helpers ----
+namespace {
+
+template <class R>
+int upload_table(void** dst, const std::vector<R>& v) {
+    if (v.empty()) { *dst = nullptr; return PF_OK; }
+    HIPCHK(hipMalloc(dst, v.size() * sizeof(R)));
+    HIPCHK(hipMemcpy(*dst, v.data(), v.size() * sizeof(R), hipMemcpyHostToDevice));
+    return PF_OK;
+}
+
+template <class R>
+std::vector<R> spacing_table(const double* src, int n, bool invert) {
+    std::vector<R> v((size_t)n);
+    for (int i = 0; i < n; i++) v[(size_t)i] = invert ? (R)(1.0 / src[i]) : (R)src[i];
+    return v;
+}
+
+// Material rows of 16: C00..C22, C33, C44, C55, dt2/rho (as the host folds
+// it, base_solver.py:443 `self.m.dt**2/self.m.P[...]`), padding.
+template <class R>
+std::vector<R> material_rows(const double* m13, int n, double dt2) {
+    std::vector<R> v((size_t)n * MAT_STRIDE, (R)0);
+    for (int r = 0; r < n; r++) {
+        for (int e = 0; e < 12; e++) v[(size_t)r * MAT_STRIDE + e] = (R)m13[r * 13 + e];
+        v[(size_t)r * MAT_STRIDE + M_COEF] = (R)(dt2 / m13[r * 13 + 12]);
+    }
+    return v;
+}
+
+int check_desc(const pf_desc* d) {
+    if (!d) return fail(PF_ERR_ARG, "desc is NULL");
+    if (d->nx < 4 || d->ny < 4 || d->nz < 4)
+        return fail(PF_ERR_ARG, "node counts must be >= 4 per axis (got %d x %d x %d)", d->nx, d->ny, d->nz);
+    if ((long long)d->nx * d->ny * d->nz > (1LL << 40)) return fail(PF_ERR_ARG, "mesh too large");
+    if (d->precision < PF_FP64 || d->precision > PF_FP32C) return fail(PF_ERR_ARG, "bad precision %d", d->precision);
+    if (d->kernel < PF_KERNEL_AUTO || d->kernel > PF_KERNEL_TILED) return fail(PF_ERR_ARG, "bad kernel %d", d->kernel);
+    if (d->n_materials < 1 || d->n_materials > 256) return fail(PF_ERR_ARG, "n_materials must be 1..256");
+    if (!d->fdx || !d->fdy || !d->fdz || !d->sdx || !d->sdy || !d->sdz) return fail(PF_ERR_ARG, "spacing table missing");
+    if (!d->materials || !d->ids) return fail(PF_ERR_ARG, "material table or ids missing");
+    if (!d->mur) return fail(PF_ERR_ARG, "Mur coefficients missing");
+    if (!d->wave || d->wave_len < 0) return fail(PF_ERR_ARG, "source table missing");
+    if (!(d->dt > 0) || !(d->dt2 > 0)) return fail(PF_ERR_ARG, "dt must be > 0");
+    const bool any = d->sponge_x || d->sponge_y || d->sponge_z;
+    const bool all = d->sponge_x && d->sponge_y && d->sponge_z;
+    if (any && !all) return fail(PF_ERR_ARG, "sponge needs all three axis tables or none");
+    for (int i = 0; i < d->nx - 1; i++) if (!(d->fdx[i] > 0)) return fail(PF_ERR_ARG, "fdx[%d] <= 0", i);
+    for (int i = 0; i < d->ny - 1; i++) if (!(d->fdy[i] > 0)) return fail(PF_ERR_ARG, "fdy[%d] <= 0", i);
+    for (int i = 0; i < d->nz - 1; i++) if (!(d->fdz[i] > 0)) return fail(PF_ERR_ARG, "fdz[%d] <= 0", i);
+    for (int r = 0; r < d->n_materials; r++)
+        if (!(d->materials[r * 13 + 12] > 0)) return fail(PF_ERR_ARG, "material %d has rho <= 0", r);
+    const long long nn = (long long)d->nx * d->ny * d->nz;
+    for (long long t = 0; t < nn; t++)
+        if (d->ids[t] >= d->n_materials) return fail(PF_ERR_ARG, "ids[%lld] = %d out of range", t, (int)d->ids[t]);
+    if (d->probe[2] > 0) {
+        const int* p = d->probe;
+        if (p[1] < 1 || p[4] < 1 || p[5] < 1 || p[0] < 0 || p[3] < 0)
+            return fail(PF_ERR_ARG, "bad probe grid");
+        if (p[0] + p[1] * (p[2] - 1) > d->nx - 1 || p[3] + p[4] * (p[5] - 1) > d->ny - 1)
+            return fail(PF_ERR_ARG, "probe grid outside the u_z surface");
+        if (d->record_every < 1 || d->probe_capacity < 1) return fail(PF_ERR_ARG, "record_every and probe_capacity must be >= 1");
+    }
+    return PF_OK;
+}
+
+}  // namespace
+
+// --------------------------------------------------------- life cycle ----
+template <class R>
+static int create_tables(pf_ctx* c, const pf_desc* d) {
+    const bool exact = d->precision == PF_FP64;
+    int rc;
+    if ((rc = upload_table(&c->fx, spacing_table<R>(d->fdx, d->nx - 1, !exact)))) return rc;
+    if ((rc = upload_table(&c->fy, spacing_table<R>(d->fdy, d->ny - 1, !exact)))) return rc;
+    if ((rc = upload_table(&c->fz, spacing_table<R>(d->fdz, d->nz - 1, !exact)))) return rc;
+    if ((rc = upload_table(&c->sx, spacing_table<R>(d->sdx, d->nx - 2, !exact)))) return rc;
+    if ((rc = upload_table(&c->sy, spacing_table<R>(d->sdy, d->ny - 2, !exact)))) return rc;
+    if ((rc = upload_table(&c->sz, spacing_table<R>(d->sdz, d->nz - 2, !exact)))) return rc;
+    if ((rc = upload_table(&c->mat, material_rows<R>(d->materials, d->n_materials, d->dt2)))) return rc;
+    if (c->sponge) {
+        if ((rc = upload_table(&c->dpx, spacing_table<R>(d->sponge_x, d->nx, false)))) return rc;
+        if ((rc = upload_table(&c->dpy, spacing_table<R>(d->sponge_y, d->ny, false)))) return rc;
+        if ((rc = upload_table(&c->dpz, spacing_table<R>(d->sponge_z, d->nz, false)))) return rc;
+    }
+    return PF_OK;
+}
+
+extern "C" int pf_create(const pf_desc* d, pf_ctx** out) {
+    if (!out) return fail(PF_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int rc = check_desc(d);
+    if (rc) return rc;
+    pf_ctx* c = new (std::nothrow) pf_ctx();
+    if (!c) return fail(PF_ERR_NOMEM, "host allocation failed");
+    rc = ctx_init(c, d);
+    if (rc) {
+        ctx_free(c);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return PF_OK;
+}
+
+int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
+    c->nx = d->nx;
+    c->ny = d->ny;
+    c->nz = d->nz;
+    c->pz = (d->nz + 31) / 32 * 32;
+    c->plane = (long long)d->ny * c->pz;
+    c->cells = (long long)d->nx * c->plane;
+    c->precision = d->precision;
+    c->kernel = d->kernel == PF_KERNEL_AUTO ? PF_KERNEL_NAIVE : d->kernel;
+    c->tile_j = d->tile_j > 0 ? d->tile_j : 4;
+    c->tile_x = d->tile_x > 0 ? d->tile_x : 0;
+    c->esz = d->precision == PF_FP64 ? 8 : 4;
+    c->sponge = d->sponge_x != nullptr;
+    c->dt = d->dt;
+    c->nmat = d->n_materials;
+    HIPCHK(hipGetDevice(&c->dev));
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+
+    // fields: two levels x three components (+ lo parts in compensated mode)
+    const size_t bytes = (size_t)c->cells * c->esz;
+    for (int l = 0; l < 2; l++)
+        for (int q = 0; q < 3; q++) {
+            HIPCHK(hipMalloc(&c->lvl[l][q], bytes));
+            HIPCHK(hipMemset(c->lvl[l][q], 0, bytes));
+            if (c->precision == PF_FP32C) {
+                HIPCHK(hipMalloc(&c->lvlo[l][q], bytes));
+                HIPCHK(hipMemset(c->lvlo[l][q], 0, bytes));
+            }
+        }
+    // material index map, padded like the fields
+    HIPCHK(hipMalloc((void**)&c->ids, (size_t)c->cells));
+    HIPCHK(hipMemset(c->ids, 0, (size_t)c->cells));
+    HIPCHK(hipMemcpy2D(c->ids, (size_t)c->pz, d->ids, (size_t)d->nz, (size_t)d->nz,
+                       (size_t)d->nx * d->ny, hipMemcpyHostToDevice));
+
+    int rc = c->esz == 8 ? create_tables<double>(c, d) : create_tables<float>(c, d);
+    if (rc) return rc;
+
+    // FP64 copies for the energy audit
+    {
+        std::vector<double> m64 = material_rows<double>(d->materials, d->n_materials, d->dt2);
+        std::vector<double> rho((size_t)d->n_materials);
+        for (int r = 0; r < d->n_materials; r++) rho[(size_t)r] = d->materials[r * 13 + 12];
+        if ((rc = upload_table(&c->mat64, m64))) return rc;
+        if ((rc = upload_table(&c->rho64, rho))) return rc;
+        if ((rc = upload_table(&c->fx64, spacing_table<double>(d->fdx, d->nx - 1, false)))) return rc;
+        if ((rc = upload_table(&c->fy64, spacing_table<double>(d->fdy, d->ny - 1, false)))) return rc;
+        if ((rc = upload_table(&c->fz64, spacing_table<double>(d->fdz, d->nz - 1, false)))) return rc;
+        if ((rc = upload_table(&c->sx64, spacing_table<double>(d->sdx, d->nx - 2, false)))) return rc;
+        if ((rc = upload_table(&c->sy64, spacing_table<double>(d->sdy, d->ny - 2, false)))) return rc;
+        if ((rc = upload_table(&c->sz64, spacing_table<double>(d->sdz, d->nz - 2, false)))) return rc;
+    }
+    for (int a = 0; a < 3; a++) {
+        const double* f = a == 0 ? d->fdx : (a == 1 ? d->fdy : d->fdz);
+        const double* s = a == 0 ? d->sdx : (a == 1 ? d->sdy : d->sdz);
+        c->f0[a] = f[0];
+        c->s0[a] = s[0];
+    }
+    double mx = 0, my = 0, mz = 0;
+    for (int i = 0; i < d->nx - 1; i++) mx += d->fdx[i];
+    for (int i = 0; i < d->ny - 1; i++) my += d->fdy[i];
+    for (int i = 0; i < d->nz - 1; i++) mz += d->fdz[i];
+    c->dV = (mx / (d->nx - 1)) * (my / (d->ny - 1)) * (mz / (d->nz - 1));
+    for (int e = 0; e < 12; e++) c->mur[e] = d->mur[e];
+
+    // source table (FP64 on the device) and step counter
+    c->wave_len = d->wave_len;
+    if (d->wave_len > 0) {
+        HIPCHK(hipMalloc((void**)&c->wave, (size_t)d->wave_len * sizeof(double)));
+        HIPCHK(hipMemcpy(c->wave, d->wave, (size_t)d->wave_len * sizeof(double), hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMalloc((void**)&c->d_step, sizeof(long long)));
+    HIPCHK(hipMemset(c->d_step, 0, sizeof(long long)));
+
+    // probes
+    memcpy(c->probe, d->probe, sizeof c->probe);
+    if (d->probe[2] > 0) {
+        c->every = d->record_every;
+        c->cap = d->probe_capacity;
+        const size_t pb = (size_t)c->cap * d->probe[2] * d->probe[5] * sizeof(double);
+        HIPCHK(hipMalloc((void**)&c->ring, pb));
+        HIPCHK(hipMemset(c->ring, 0, pb));
+        HIPCHK(hipMalloc((void**)&c->d_frame, sizeof(long long)));
+        HIPCHK(hipMemset(c->d_frame, 0, sizeof(long long)));
+    }
+    HIPCHK(hipMalloc((void**)&c->e_out, 2 * sizeof(double)));
+    c->e_blocks = 1024;
+    HIPCHK(hipMalloc((void**)&c->e_partials, 2 * sizeof(double) * (size_t)c->e_blocks));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
+void pf::ctx_free(pf_ctx* c) {
+    auto F = [](void* p) { if (p) (void)hipFree(p); };
+    for (int l = 0; l < 2; l++)
+        for (int q = 0; q < 3; q++) { F(c->lvl[l][q]); F(c->lvlo[l][q]); }
+    F(c->ids);
+    F(c->mat); F(c->fx); F(c->fy); F(c->fz); F(c->sx); F(c->sy); F(c->sz);
+    F(c->dpx); F(c->dpy); F(c->dpz);
+    F(c->mat64); F(c->rho64); F(c->fx64); F(c->fy64); F(c->fz64); F(c->sx64); F(c->sy64); F(c->sz64);
+    F(c->wave); F(c->d_step); F(c->ring); F(c->d_frame); F(c->e_out); F(c->e_partials);
+    F(c->scratch);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+extern "C" int pf_destroy(pf_ctx* c) {
+    if (!c) return PF_OK;
+    if (c->batch) return fail(PF_ERR_STATE, "context is still a member of a batch");
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    ctx_free(c);
+    delete c;
+    return PF_OK;
+}
+
+// ----------------------------------------------------------- stepping ----
+template <class R>
+Mesh<R> pf::make_view(const pf_ctx* c, int cur) {
+    Mesh<R> m;
+    memset(&m, 0, sizeof m);
+    m.nx = c->nx; m.ny = c->ny; m.nz = c->nz; m.pz = c->pz; m.plane = c->plane;
+    const int nxt = 1 - cur;
+    m.c0 = (const R*)c->lvl[cur][0]; m.c1 = (const R*)c->lvl[cur][1]; m.c2 = (const R*)c->lvl[cur][2];
+    m.n0 = (R*)c->lvl[nxt][0]; m.n1 = (R*)c->lvl[nxt][1]; m.n2 = (R*)c->lvl[nxt][2];
+    m.cl0 = (const R*)c->lvlo[cur][0]; m.cl1 = (const R*)c->lvlo[cur][1]; m.cl2 = (const R*)c->lvlo[cur][2];
+    m.nl0 = (R*)c->lvlo[nxt][0]; m.nl1 = (R*)c->lvlo[nxt][1]; m.nl2 = (R*)c->lvlo[nxt][2];
+    m.ids = c->ids;
+    m.mat = (const R*)c->mat;
+    m.fx = (const R*)c->fx; m.fy = (const R*)c->fy; m.fz = (const R*)c->fz;
+    m.sx = (const R*)c->sx; m.sy = (const R*)c->sy; m.sz = (const R*)c->sz;
+    const bool exact = c->precision == PF_FP64;
+    m.fx0 = (R)(exact ? c->f0[0] : 1.0 / c->f0[0]);
+    m.fy0 = (R)(exact ? c->f0[1] : 1.0 / c->f0[1]);
+    m.fz0 = (R)(exact ? c->f0[2] : 1.0 / c->f0[2]);
+    m.sx0 = (R)(exact ? c->s0[0] : 1.0 / c->s0[0]);
+    m.sy0 = (R)(exact ? c->s0[1] : 1.0 / c->s0[1]);
+    m.sz0 = (R)(exact ? c->s0[2] : 1.0 / c->s0[2]);
+    m.dpx = (const R*)c->dpx; m.dpy = (const R*)c->dpy; m.dpz = (const R*)c->dpz;
+    for (int e = 0; e < 12; e++) {
+        m.mur[e] = (R)c->mur[e];
+        m.mur_lo[e] = (R)(c->mur[e] - (double)m.mur[e]);
+    }
+    m.wave = c->wave;
+    m.step_base = nullptr;
+    m.step_off = 0;
+    m.nsteps = c->wave_len;
+    return m;
+}
+
+template Mesh<float> pf::make_view<float>(const pf_ctx*, int);
+template Mesh<double> pf::make_view<double>(const pf_ctx*, int);
+
+// Launch the sweep of one step of mesh view m on stream s.
+template <class R, int MODE>
+static int launch_sweep_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
+    if (c->kernel == PF_KERNEL_TILED && MODE != MODE_FP64) {
+        return launch_tiled<R, MODE>(c, m, s);
+    }
+    dim3 blk(64, 4, 1);
+    dim3 grd((unsigned)((c->nz + 63) / 64), (unsigned)((c->ny + 3) / 4), (unsigned)c->nx);
+    if (c->sponge) hipLaunchKernelGGL((k_sweep_naive<R, MODE, true>), grd, blk, 0, s, m);
+    else hipLaunchKernelGGL((k_sweep_naive<R, MODE, false>), grd, blk, 0, s, m);
+    HIPCHK(hipGetLastError());
+    return PF_OK;
+}
+
+template <class R, int MODE>
+static int launch_faces_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
+    long long n = face_total<R>(c->nx, c->ny, c->nz);
+    unsigned g = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL((k_faces<R, MODE>), dim3(g), dim3(256), 0, s, m);
+    HIPCHK(hipGetLastError());
+    return PF_OK;
+}
+
+template <class R>
+static int launch_probe_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
+    if (c->probe[2] <= 0) return PF_OK;
+    // newest level after this step = the n* buffers of the view
+    hipLaunchKernelGGL((k_probe<R>), dim3(1), dim3(256), 0, s, (const R*)m.n2, (const R*)m.nl2, 0,
+                       c->plane, c->pz, c->probe[0], c->probe[1], c->probe[2], c->probe[3], c->probe[4],
+                       c->probe[5], c->ring, c->cap, c->d_frame, m.step_base, m.step_off, c->every,
+                       m.nsteps);
+    HIPCHK(hipGetLastError());
+    return PF_OK;
+}
+
+template <class R, int MODE>
+static int step_once_t(pf_ctx* c, hipStream_t s, const long long* base, long long off,
+                       hipEvent_t* ev /* 3 or null */) {
+    Mesh<R> m = make_view<R>(c, c->cur);
+    m.step_base = base;
+    m.step_off = off;
+    int rc;
+    if (ev) HIPCHK(hipEventRecord(ev[0], s));
+    if ((rc = launch_sweep_t<R, MODE>(c, m, s))) return rc;
+    if (ev) HIPCHK(hipEventRecord(ev[1], s));
+    if ((rc = launch_faces_t<R, MODE>(c, m, s))) return rc;
+    if (ev) HIPCHK(hipEventRecord(ev[2], s));
+    if ((rc = launch_probe_t<R>(c, m, s))) return rc;
+    c->cur = 1 - c->cur;
+    return PF_OK;
+}
+
+int pf::step_once(pf_ctx* c, hipStream_t s, const long long* base, long long off, hipEvent_t* ev) {
+    switch (c->precision) {
+        case PF_FP64: return step_once_t<double, MODE_FP64>(c, s, base, off, ev);
+        case PF_FP32: return step_once_t<float, MODE_FP32>(c, s, base, off, ev);
+        default: return step_once_t<float, MODE_FP32C>(c, s, base, off, ev);
+    }
+}
+
+extern "C" int pf_step(pf_ctx* c, int64_t nsteps) {
+    if (!c) return fail(PF_ERR_ARG, "ctx is NULL");
+    if (c->batch) return fail(PF_ERR_STATE, "context is stepped by its batch");
+    if (nsteps < 0) return fail(PF_ERR_ARG, "nsteps < 0");
+    if (c->step + nsteps > c->wave_len)
+        return fail(PF_ERR_RANGE, "step %lld + %lld beyond the source table (%lld)", (long long)c->step,
+                    (long long)nsteps, (long long)c->wave_len);
+    for (int64_t s = 0; s < nsteps; s++) {
+        int rc = step_once(c, c->stream, nullptr, c->step, nullptr);
+        if (rc) return rc;
+        c->step++;
+    }
+    return PF_OK;
+}
+
+extern "C" int pf_sync(pf_ctx* c) {
+    if (!c) return fail(PF_ERR_ARG, "ctx is NULL");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
+extern "C" int pf_steps_done(pf_ctx* c, int64_t* out) {
+    if (!c || !out) return fail(PF_ERR_ARG, "NULL argument");
+    *out = c->step;
+    return PF_OK;
+}
+
+// ---------------------------------------------------- field transfer ----
+static void extents(const pf_ctx* c, int q, int& ex, int& ey, int& ez) {
+    ex = c->nx - (q == 0);
+    ey = c->ny - (q == 1);
+    ez = c->nz - (q == 2);
+}
+
+static int ensure_scratch(pf_ctx* c, size_t bytes) {
+    if (c->scratch_bytes >= bytes) return PF_OK;
+    if (c->scratch) HIPCHK(hipFree(c->scratch));
+    c->scratch = nullptr;
+    c->scratch_bytes = 0;
+    HIPCHK(hipMalloc(&c->scratch, bytes));
+    c->scratch_bytes = bytes;
+    return PF_OK;
+}
+
+extern "C" int pf_get_field(pf_ctx* c, int32_t which, double* host, int64_t count) {
+    if (!c || !host) return fail(PF_ERR_ARG, "NULL argument");
+    if (which < 0 || which > 5) return fail(PF_ERR_ARG, "bad field %d", which);
+    if (c->batch) HIPCHK(hipStreamSynchronize(c->batch_stream));
+    const int q = which % 3;
+    const int lvl = which < 3 ? c->cur : 1 - c->cur;
+    int ex, ey, ez;
+    extents(c, q, ex, ey, ez);
+    const long long n = (long long)ex * ey * ez;
+    if (count != n) return fail(PF_ERR_ARG, "field %d has %lld elements, got %lld", which, n, (long long)count);
+    int rc = ensure_scratch(c, (size_t)n * sizeof(double));
+    if (rc) return rc;
+    // the previous level of uz holds the source value on its wall line, as
+    // the reference's u_old does after time_step
+    int subst = (which == PF_UZ_OLD && c->step > 0) ? 1 : 0;
+    double w = 0.0;
+    if (subst) HIPCHK(hipMemcpy(&w, c->wave + (c->step - 1), sizeof(double), hipMemcpyDeviceToHost));
+    unsigned g = (unsigned)((n + 255) / 256);
+    if (c->esz == 8)
+        hipLaunchKernelGGL((k_pack<double>), dim3(g), dim3(256), 0, c->stream, (const double*)c->lvl[lvl][q],
+                           (const double*)nullptr, c->plane, c->pz, ex, ey, ez, (double*)c->scratch, subst, w);
+    else
+        hipLaunchKernelGGL((k_pack<float>), dim3(g), dim3(256), 0, c->stream, (const float*)c->lvl[lvl][q],
+                           (const float*)c->lvlo[lvl][q], c->plane, c->pz, ex, ey, ez, (double*)c->scratch,
+                           subst, w);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(host, c->scratch, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
+extern "C" int pf_set_field(pf_ctx* c, int32_t which, const double* host, int64_t count) {
+    if (!c || !host) return fail(PF_ERR_ARG, "NULL argument");
+    if (which < 0 || which > 5) return fail(PF_ERR_ARG, "bad field %d", which);
+    if (c->batch) return fail(PF_ERR_STATE, "context is stepped by its batch");
+    const int q = which % 3;
+    const int lvl = which < 3 ? c->cur : 1 - c->cur;
+    int ex, ey, ez;
+    extents(c, q, ex, ey, ez);
+    const long long n = (long long)ex * ey * ez;
+    if (count != n) return fail(PF_ERR_ARG, "field %d has %lld elements, got %lld", which, n, (long long)count);
+    int rc = ensure_scratch(c, (size_t)n * sizeof(double));
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->scratch, host, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    unsigned g = (unsigned)((n + 255) / 256);
+    if (c->esz == 8)
+        hipLaunchKernelGGL((k_unpack<double>), dim3(g), dim3(256), 0, c->stream, (double*)c->lvl[lvl][q],
+                           (double*)nullptr, c->plane, c->pz, ex, ey, ez, (const double*)c->scratch);
+    else
+        hipLaunchKernelGGL((k_unpack<float>), dim3(g), dim3(256), 0, c->stream, (float*)c->lvl[lvl][q],
+                           (float*)c->lvlo[lvl][q], c->plane, c->pz, ex, ey, ez, (const double*)c->scratch);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
+// ------------------------------------------------------------- probes ----
+extern "C" int pf_get_probes(pf_ctx* c, double* host, int64_t cap_frames, int64_t* nframes) {
+    if (!c || !nframes) return fail(PF_ERR_ARG, "NULL argument");
+    *nframes = 0;
+    if (c->probe[2] <= 0) return PF_OK;
+    hipStream_t s = c->batch ? c->batch_stream : c->stream;
+    HIPCHK(hipStreamSynchronize(s));
+    long long f = 0;
+    HIPCHK(hipMemcpy(&f, c->d_frame, sizeof f, hipMemcpyDeviceToHost));
+    const long long per = (long long)c->probe[2] * c->probe[5];
+    const long long have = f < c->cap ? f : c->cap;
+    const long long first = f - have;  // oldest frame still in the ring
+    const long long take = have < cap_frames ? have : cap_frames;
+    if (take > 0 && !host) return fail(PF_ERR_ARG, "host is NULL");
+    for (long long t = 0; t < take; t++) {
+        long long slot = (first + t) % c->cap;
+        HIPCHK(hipMemcpy(host + t * per, c->ring + slot * per, (size_t)per * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    *nframes = take;
+    return PF_OK;
+}
+
+// ------------------------------------------------------------- energy ----
+template <class R>
+__global__ void k_widen(const R* __restrict__ hi, const R* __restrict__ lo, double* __restrict__ out, long long n) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double v = (double)hi[t];
+    if (lo) v += (double)lo[t];
+    out[t] = v;
+}
+
+extern "C" int pf_energy(pf_ctx* c, double* out) {
+    if (!c || !out) return fail(PF_ERR_ARG, "NULL argument");
+    hipStream_t s = c->batch ? c->batch_stream : c->stream;
+    EnergyView ev;
+    memset(&ev, 0, sizeof ev);
+    const int newest = c->cur, prev = 1 - c->cur;
+    if (c->esz == 8) {
+        ev.m = make_view<double>(c, newest);
+    } else {
+        int rc = ensure_scratch(c, (size_t)c->cells * sizeof(double) * 6);
+        if (rc) return rc;
+        double* w = (double*)c->scratch;
+        unsigned g = (unsigned)((c->cells + 255) / 256);
+        for (int l = 0; l < 2; l++)
+            for (int q = 0; q < 3; q++) {
+                int lv = l == 0 ? newest : prev;
+                hipLaunchKernelGGL((k_widen<float>), dim3(g), dim3(256), 0, s, (const float*)c->lvl[lv][q],
+                                   (const float*)c->lvlo[lv][q], w + (size_t)(3 * l + q) * c->cells, c->cells);
+            }
+        HIPCHK(hipGetLastError());
+        Mesh<double>& m = ev.m;
+        memset(&m, 0, sizeof m);
+        m.nx = c->nx; m.ny = c->ny; m.nz = c->nz; m.pz = c->pz; m.plane = c->plane;
+        m.c0 = w; m.c1 = w + c->cells; m.c2 = w + 2 * c->cells;
+        m.n0 = w + 3 * c->cells; m.n1 = w + 4 * c->cells; m.n2 = w + 5 * c->cells;
+        m.ids = c->ids;
+    }
+    Mesh<double>& m = ev.m;
+    m.mat = (const double*)c->mat64;
+    m.fx = (const double*)c->fx64; m.fy = (const double*)c->fy64; m.fz = (const double*)c->fz64;
+    m.sx = (const double*)c->sx64; m.sy = (const double*)c->sy64; m.sz = (const double*)c->sz64;
+    m.fx0 = c->f0[0]; m.fy0 = c->f0[1]; m.fz0 = c->f0[2];
+    m.sx0 = c->s0[0]; m.sy0 = c->s0[1]; m.sz0 = c->s0[2];
+    ev.rho = (const double*)c->rho64;
+    ev.dt = c->dt;
+    hipLaunchKernelGGL(k_energy_partial, dim3(c->e_blocks), dim3(256), 0, s, ev, c->e_partials);
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, s, (const double*)c->e_partials, c->e_blocks, c->e_out);
+    HIPCHK(hipGetLastError());
+    double h[2];
+    HIPCHK(hipMemcpyAsync(h, c->e_out, sizeof h, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    *out = 0.5 * c->dV * (h[0] + h[1]);
+    return PF_OK;
+}
+
+// ------------------------------------------------------------ profile ----
+extern "C" int pf_profile(pf_ctx* c, int64_t nsteps, double* sweep_ms, double* face_ms) {
+    if (!c || !sweep_ms || !face_ms) return fail(PF_ERR_ARG, "NULL argument");
+    if (c->batch) return fail(PF_ERR_STATE, "context is stepped by its batch");
+    if (nsteps < 1) return fail(PF_ERR_ARG, "nsteps < 1");
+    if (c->step + nsteps > c->wave_len) return fail(PF_ERR_RANGE, "beyond the source table");
+    std::vector<hipEvent_t> ev((size_t)nsteps * 3);
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    int rc = PF_OK;
+    for (int64_t s = 0; s < nsteps && !rc; s++) {
+        rc = step_once(c, c->stream, nullptr, c->step, &ev[(size_t)s * 3]);
+        if (!rc) c->step++;
+    }
+    if (!rc) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        double sw = 0, fc = 0;
+        for (int64_t s = 0; s < nsteps; s++) {
+            float a = 0, b = 0;
+            HIPCHK(hipEventElapsedTime(&a, ev[(size_t)s * 3], ev[(size_t)s * 3 + 1]));
+            HIPCHK(hipEventElapsedTime(&b, ev[(size_t)s * 3 + 1], ev[(size_t)s * 3 + 2]));
+            sw += a;
+            fc += b;
+        }
+        *sweep_ms = sw / nsteps;
+        *face_ms = fc / nsteps;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return rc;
+}

@@ -1,0 +1,491 @@
+// Device side of the FDTD step: mesh view, stress/displacement stencils, the
+// naive sweep, the boundary-face kernel, probes, energy audit, pack/unpack.
+//
+// Layout in HBM (one mesh): each displacement component is a dense
+// (nx, ny, pz) array, z contiguous, pz = nz rounded up to 32 elements, for
+// two time levels (ping-pong).  Cells outside a component's reference extent
+// (ux has nx-1 planes, uy ny-1 rows, uz nz-1 columns) are allocated, never
+// written and stay zero.  Material: one uint8 index per node in the same
+// padded layout, and a 16-value row per material.
+//
+// Semantics of one step (what the reference does in base_solver.py:245-256):
+//   1. the wall line uz[0, :, 0] takes the source value w(tt); here the value
+//      is substituted on every read of that line instead of being stored;
+//   2. stresses from u (T is never stored: each stencil recomputes it);
+//   3. "stage I": the interior and free-surface displacement update;
+//   4. the four Mur faces, x=max, y=0, y=max, z=max, each reading the plane
+//      one cell inside as updated so far and overwriting shared edges;
+//   5. rotation: the new level becomes current (pointer swap).
+// Stage 4 is resolved per cell: the last face that writes a cell fixes its
+// value as a chain  face(... face(stageI(inner cell)))  of at most four maps,
+// and every stage-I value a chain reads lies strictly inside all faces, where
+// the sweep kernel has already written it.  So the sweep writes all non-face
+// cells, then one small kernel writes all face cells.  Cells of uy/uz on the
+// x = 0 wall that the reference never updates are written as 0, which is
+// what the reference holds there (the source line included, whose stored
+// value after a step is the never-written 0 of u_new).
+#pragma once
+#include "pf_formulas.hpp"
+
+namespace pf {
+
+template <class R>
+struct Mesh {
+    int nx, ny, nz, pz;
+    long long plane;  // ny * pz
+    // level n (current), hi part
+    const R* __restrict__ c0;
+    const R* __restrict__ c1;
+    const R* __restrict__ c2;
+    // level n-1 on entry, level n+1 on exit, hi part
+    R* n0;
+    R* n1;
+    R* n2;
+    // lo parts (PF_FP32C only)
+    const R* __restrict__ cl0;
+    const R* __restrict__ cl1;
+    const R* __restrict__ cl2;
+    R* nl0;
+    R* nl1;
+    R* nl2;
+    const uint8_t* __restrict__ ids;
+    const R* __restrict__ mat;
+    // spacing tables: FP64 = spacings, FP32 = inverse spacings
+    const R* __restrict__ fx;
+    const R* __restrict__ fy;
+    const R* __restrict__ fz;
+    const R* __restrict__ sx;
+    const R* __restrict__ sy;
+    const R* __restrict__ sz;
+    R fx0, fy0, fz0, sx0, sy0, sz0;
+    const R* __restrict__ dpx;  // sponge terms per axis (null: none)
+    const R* __restrict__ dpy;
+    const R* __restrict__ dpz;
+    R mur[12];
+    R mur_lo[12];  // FP32C: low parts of the Mur coefficients
+    const double* __restrict__ wave;
+    const long long* __restrict__ step_base;  // device step counter or null
+    long long step_off;                       // tt = *step_base + step_off
+    long long nsteps;                         // step budget (batch members)
+};
+
+template <class R>
+PF_DEV long long at(const Mesh<R>& m, int i, int j, int k) {
+    return (long long)i * m.plane + (long long)j * m.pz + k;
+}
+
+template <class R>
+PF_DEV long long step_index(const Mesh<R>& m) {
+    return (m.step_base ? *m.step_base : 0) + m.step_off;
+}
+
+// Component extents: last valid index per axis.
+template <int Q> struct Ext;
+template <> struct Ext<0> { static PF_DEV int Lx(int nx) { return nx - 2; } static PF_DEV int Ly(int ny) { return ny - 1; } static PF_DEV int Lz(int nz) { return nz - 1; } };
+template <> struct Ext<1> { static PF_DEV int Lx(int nx) { return nx - 1; } static PF_DEV int Ly(int ny) { return ny - 2; } static PF_DEV int Lz(int nz) { return nz - 1; } };
+template <> struct Ext<2> { static PF_DEV int Lx(int nx) { return nx - 1; } static PF_DEV int Ly(int ny) { return ny - 1; } static PF_DEV int Lz(int nz) { return nz - 2; } };
+
+// ---------------------------------------------------------------- loads ----
+template <class R> PF_DEV R lux(const Mesh<R>& m, int i, int j, int k) { return m.c0[at(m, i, j, k)]; }
+template <class R> PF_DEV R luy(const Mesh<R>& m, int i, int j, int k) { return m.c1[at(m, i, j, k)]; }
+template <class R> PF_DEV R luz(const Mesh<R>& m, int i, int j, int k, R w) {
+    R v = m.c2[at(m, i, j, k)];
+    return (i == 0 && k == 0) ? w : v;
+}
+template <class R> PF_DEV const R* matp(const Mesh<R>& m, int i, int j, int k) {
+    return m.mat + MAT_STRIDE * (int)m.ids[at(m, i, j, k)];
+}
+
+// ------------------------------------------------------------- stresses ----
+// Each returns the value the reference's T array holds at (i,j,k) after
+// update_T + apply_T_tfbc, and 0 where the reference never writes it.
+template <bool EXACT, class R>
+PF_DEV void tn(const Mesh<R>& m, int i, int j, int k, R w, R& t1, R& t2, R& t3) {
+    t1 = t2 = t3 = R(0);
+    if (i < 1 || i > m.nx - 2 || j < 1 || j > m.ny - 2 || k < 0 || k > m.nz - 2) return;
+    const R* c = matp(m, i, j, k);
+    R dux = lux(m, i, j, k) - lux(m, i - 1, j, k);
+    R duy = luy(m, i, j, k) - luy(m, i, j - 1, k);
+    if (k >= 1) {
+        R duz = luz(m, i, j, k, w) - luz(m, i, j, k - 1, w);
+        normal3<EXACT>(c, dux, duy, duz, m.sx[i - 1], m.sy[j - 1], m.sz[k - 1], t1, t2, t3);
+    } else {
+        normal3_surface<EXACT>(c, dux, duy, luz(m, i, j, 0, w), m.sx0, m.sy0, m.sz0, t1, t2, t3);
+    }
+}
+
+template <bool EXACT, class R>
+PF_DEV R t4(const Mesh<R>& m, int i, int j, int k, R w) {
+    if (i < 1 || i > m.nx - 2 || j < 0 || j > m.ny - 2 || k < 0 || k > m.nz - 2) return R(0);
+    if (k >= 1) {
+        R c = matp(m, i, j + 1, k + 1)[M_C33];
+        return shear<EXACT>(c, luy(m, i, j, k + 1) - luy(m, i, j, k), m.fz[k],
+                            luz(m, i, j + 1, k, w) - luz(m, i, j, k, w), m.fy[j]);
+    }
+    R c = matp(m, i, j + 1, 0)[M_C33];
+    return shear<EXACT>(c, luy(m, i, j, 1) - luy(m, i, j, 0), m.fy0,
+                        luz(m, i, j + 1, 0, w) - luz(m, i, j, 0, w), m.fz0);
+}
+
+template <bool EXACT, class R>
+PF_DEV R t5(const Mesh<R>& m, int i, int j, int k, R w) {
+    if (i < 0 || i > m.nx - 2 || j < 1 || j > m.ny - 2 || k < 0 || k > m.nz - 2) return R(0);
+    if (k >= 1) {
+        R c = matp(m, i + 1, j, k + 1)[M_C44];
+        return shear<EXACT>(c, lux(m, i, j, k + 1) - lux(m, i, j, k), m.fz[k],
+                            luz(m, i + 1, j, k, w) - luz(m, i, j, k, w), m.fx[i]);
+    }
+    R c = matp(m, i + 1, j, 0)[M_C44];
+    return shear<EXACT>(c, lux(m, i, j, 1) - lux(m, i, j, 0), m.fx0,
+                        luz(m, i + 1, j, 0, w) - luz(m, i, j, 0, w), m.fz0);
+}
+
+template <bool EXACT, class R>
+PF_DEV R t6(const Mesh<R>& m, int i, int j, int k, R w) {
+    if (i < 0 || i > m.nx - 2 || j < 0 || j > m.ny - 2 || k < 0 || k > m.nz - 2) return R(0);
+    (void)w;
+    if (k >= 1) {
+        R c = matp(m, i + 1, j + 1, k)[M_C55];
+        return shear<EXACT>(c, lux(m, i, j + 1, k) - lux(m, i, j, k), m.fy[j],
+                            luy(m, i + 1, j, k) - luy(m, i, j, k), m.fx[i]);
+    }
+    R c = matp(m, i + 1, j + 1, 0)[M_C55];
+    return shear<EXACT>(c, lux(m, i, j + 1, 0) - lux(m, i, j, 0), m.fx0,
+                        luy(m, i + 1, j, 0) - luy(m, i, j, 0), m.fz0);
+}
+
+template <class R>
+PF_DEV R damp_at(const Mesh<R>& m, int i, int j, int k) {
+    return m.dpx[i] + m.dpy[j] + m.dpz[k];
+}
+
+// Stage-I bracket ("div") and coefficient of each component at a non-face
+// cell (base_solver.py:441-463 for k >= 1, :495-517 for k = 0).
+template <bool EXACT, class R>
+PF_DEV void stage_ux(const Mesh<R>& m, int i, int j, int k, R w, R& coef, R& dv) {
+    R a1, a2, a3, b1, b2, b3;
+    tn<EXACT>(m, i + 1, j, k, w, a1, a2, a3);
+    tn<EXACT>(m, i, j, k, w, b1, b2, b3);
+    R d6 = t6<EXACT>(m, i, j, k, w) - t6<EXACT>(m, i, j - 1, k, w);
+    if (k >= 1) {
+        R d5 = t5<EXACT>(m, i, j, k, w) - t5<EXACT>(m, i, j, k - 1, w);
+        dv = div3<EXACT>(a1 - b1, m.fx[i], d6, m.sy[j - 1], d5, m.sz[k - 1]);
+        coef = matp(m, i + 1, j, k)[M_COEF];
+    } else {
+        R d5 = t5<EXACT>(m, i, j, 0, w) - R(0);
+        dv = div3<EXACT>(a1 - b1, m.fx0, d6, m.sy0, d5, m.sz0);
+        coef = matp(m, i + 1, j, 0)[M_COEF];
+    }
+}
+
+template <bool EXACT, class R>
+PF_DEV void stage_uy(const Mesh<R>& m, int i, int j, int k, R w, R& coef, R& dv) {
+    R d6 = t6<EXACT>(m, i, j, k, w) - t6<EXACT>(m, i - 1, j, k, w);
+    R a1, a2, a3, b1, b2, b3;
+    tn<EXACT>(m, i, j + 1, k, w, a1, a2, a3);
+    tn<EXACT>(m, i, j, k, w, b1, b2, b3);
+    if (k >= 1) {
+        R d4 = t4<EXACT>(m, i, j, k, w) - t4<EXACT>(m, i, j, k - 1, w);
+        dv = div3<EXACT>(d6, m.sx[i - 1], a2 - b2, m.fy[j], d4, m.sz[k - 1]);
+        coef = matp(m, i, j + 1, k)[M_COEF];
+    } else {
+        R d4 = t4<EXACT>(m, i, j, 0, w) - R(0);
+        dv = div3<EXACT>(d6, m.sx0, a2 - b2, m.fy0, d4, m.sz0);
+        coef = matp(m, i, j + 1, 0)[M_COEF];
+    }
+}
+
+template <bool EXACT, class R>
+PF_DEV void stage_uz(const Mesh<R>& m, int i, int j, int k, R w, R& coef, R& dv) {
+    R d5 = t5<EXACT>(m, i, j, k, w) - t5<EXACT>(m, i - 1, j, k, w);
+    R d4 = t4<EXACT>(m, i, j, k, w) - t4<EXACT>(m, i, j - 1, k, w);
+    R a1, a2, a3, b1, b2, b3;
+    tn<EXACT>(m, i, j, k + 1, w, a1, a2, a3);
+    tn<EXACT>(m, i, j, k, w, b1, b2, b3);
+    if (k >= 1) {
+        dv = div3<EXACT>(d5, m.sx[i - 1], d4, m.sy[j - 1], a3 - b3, m.fz[k]);
+        coef = matp(m, i, j, k + 1)[M_COEF];
+    } else {
+        dv = div3_uz_surface<EXACT>(d5, m.sx0, d4, m.sy0, a3, b3, m.fz0);
+        coef = matp(m, i, j, 0)[M_COEF];  // reference samples rho at k = 0 here
+    }
+}
+
+// Write the stage-I result of one component at a non-face cell.
+template <int MODE, bool SPONGE, class R>
+PF_DEV void store_leap(const Mesh<R>& m, R* nh, R* nl, const R* ch, const R* cl, long long id,
+                       R coef, R dv, R s) {
+    if constexpr (MODE == MODE_FP32C) {
+        float f = coef * dv;
+        float rh, rl;
+        leap_comp<SPONGE>(ch[id], cl[id], nh[id], nl[id], f, s, rh, rl);
+        nh[id] = rh;
+        nl[id] = rl;
+    } else {
+        nh[id] = leap<MODE == MODE_FP64, SPONGE>(ch[id], nh[id], coef, dv, s);
+    }
+}
+
+// ------------------------------------------------------- naive sweep ----
+// One thread per node; every stress a stencil needs is recomputed from u.
+template <class R, int MODE, bool SPONGE>
+__global__ __launch_bounds__(256) void k_sweep_naive(Mesh<R> m) {
+    constexpr bool EXACT = MODE == MODE_FP64;
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k >= m.nz || j >= m.ny || i >= m.nx) return;
+    const long long tt = step_index(m);
+    if (tt >= m.nsteps) return;
+    const R w = (R)m.wave[tt];
+    const long long id = at(m, i, j, k);
+    const R s = SPONGE ? damp_at(m, i, j, k) : R(0);
+    // ux: non-face cells i <= nx-3, 1 <= j <= ny-2, k <= nz-2
+    if (i <= m.nx - 3 && j >= 1 && j <= m.ny - 2 && k <= m.nz - 2) {
+        R coef, dv;
+        stage_ux<EXACT>(m, i, j, k, w, coef, dv);
+        store_leap<MODE, SPONGE>(m, m.n0, m.nl0, m.c0, m.cl0, id, coef, dv, s);
+    }
+    // uy: non-face cells i <= nx-2, 1 <= j <= ny-3, k <= nz-2; wall i = 0 -> 0
+    if (i <= m.nx - 2 && j >= 1 && j <= m.ny - 3 && k <= m.nz - 2) {
+        if (i == 0) {
+            m.n1[id] = R(0);
+            if constexpr (MODE == MODE_FP32C) m.nl1[id] = R(0);
+        } else {
+            R coef, dv;
+            stage_uy<EXACT>(m, i, j, k, w, coef, dv);
+            store_leap<MODE, SPONGE>(m, m.n1, m.nl1, m.c1, m.cl1, id, coef, dv, s);
+        }
+    }
+    // uz: non-face cells i <= nx-2, 1 <= j <= ny-2, k <= nz-3; wall i = 0 -> 0
+    if (i <= m.nx - 2 && j >= 1 && j <= m.ny - 2 && k <= m.nz - 3) {
+        if (i == 0) {
+            m.n2[id] = R(0);
+            if constexpr (MODE == MODE_FP32C) m.nl2[id] = R(0);
+        } else {
+            R coef, dv;
+            stage_uz<EXACT>(m, i, j, k, w, coef, dv);
+            store_leap<MODE, SPONGE>(m, m.n2, m.nl2, m.c2, m.cl2, id, coef, dv, s);
+        }
+    }
+}
+
+// ----------------------------------------------------------- faces ----
+// A value in flight through the face chain: hi (+ lo in compensated mode).
+template <class R> struct V { R h, l; };
+
+template <int Q, class R>
+PF_DEV V<R> cur_q(const Mesh<R>& m, int i, int j, int k, R w) {
+    long long id = at(m, i, j, k);
+    const R* c = Q == 0 ? m.c0 : (Q == 1 ? m.c1 : m.c2);
+    const R* cl = Q == 0 ? m.cl0 : (Q == 1 ? m.cl1 : m.cl2);
+    V<R> v;
+    v.h = c[id];
+    v.l = cl ? cl[id] : R(0);
+    if (Q == 2 && i == 0 && k == 0) { v.h = w; v.l = R(0); }
+    return v;
+}
+
+template <int Q, class R>
+PF_DEV V<R> new_q(const Mesh<R>& m, int i, int j, int k) {
+    long long id = at(m, i, j, k);
+    const R* n = Q == 0 ? m.n0 : (Q == 1 ? m.n1 : m.n2);
+    const R* nl = Q == 0 ? m.nl0 : (Q == 1 ? m.nl1 : m.nl2);
+    V<R> v;
+    v.h = n[id];
+    v.l = nl ? nl[id] : R(0);
+    return v;
+}
+
+// a + c * (b - d), the reference's `q[..] + c*(q_new[..] - q[..])`
+template <int MODE, class R>
+PF_DEV V<R> mur_map(V<R> a, R ch, R cl, V<R> b, V<R> d) {
+    V<R> r;
+    if constexpr (MODE == MODE_FP32C) {
+        mur_comp(a.h, a.l, ch, cl, b.h, b.l, d.h, d.l, r.h, r.l);
+    } else {
+        r.h = a.h + ch * (b.h - d.h);
+        r.l = R(0);
+    }
+    return r;
+}
+
+// Value after stage I and the x = max face (face index 0).
+template <int Q, int MODE, class R>
+PF_DEV V<R> after_x(const Mesh<R>& m, int i, int j, int k, R w) {
+    const int Lx = Ext<Q>::Lx(m.nx);
+    if (i == Lx)
+        return mur_map<MODE>(cur_q<Q>(m, Lx - 1, j, k, w), m.mur[0 + Q], m.mur_lo[0 + Q],
+                             new_q<Q>(m, Lx - 1, j, k), cur_q<Q>(m, Lx, j, k, w));
+    return new_q<Q>(m, i, j, k);
+}
+// ... then the y = 0 face (face index 1)
+template <int Q, int MODE, class R>
+PF_DEV V<R> after_y0(const Mesh<R>& m, int i, int j, int k, R w) {
+    if (j == 0)
+        return mur_map<MODE>(cur_q<Q>(m, i, 1, k, w), m.mur[3 + Q], m.mur_lo[3 + Q],
+                             after_x<Q, MODE>(m, i, 1, k, w), cur_q<Q>(m, i, 0, k, w));
+    return after_x<Q, MODE>(m, i, j, k, w);
+}
+// ... then the y = max face (face index 2)
+template <int Q, int MODE, class R>
+PF_DEV V<R> after_y1(const Mesh<R>& m, int i, int j, int k, R w) {
+    const int Ly = Ext<Q>::Ly(m.ny);
+    if (j == Ly)
+        return mur_map<MODE>(cur_q<Q>(m, i, Ly - 1, k, w), m.mur[6 + Q], m.mur_lo[6 + Q],
+                             after_y0<Q, MODE>(m, i, Ly - 1, k, w), cur_q<Q>(m, i, Ly, k, w));
+    return after_y0<Q, MODE>(m, i, j, k, w);
+}
+// ... then the z = max face (face index 3): the final value.
+template <int Q, int MODE, class R>
+PF_DEV V<R> final_value(const Mesh<R>& m, int i, int j, int k, R w) {
+    const int Lz = Ext<Q>::Lz(m.nz);
+    if (k == Lz)
+        return mur_map<MODE>(cur_q<Q>(m, i, j, Lz - 1, w), m.mur[9 + Q], m.mur_lo[9 + Q],
+                             after_y1<Q, MODE>(m, i, j, Lz - 1, w), cur_q<Q>(m, i, j, Lz, w));
+    return after_y1<Q, MODE>(m, i, j, k, w);
+}
+
+// Face cells of component Q, enumerated once each:
+//   X  : i = Lx,            all j, all k          (Ey * Ez)
+//   Y0 : j = 0,  i < Lx,    all k                 (Lx * Ez)
+//   Y1 : j = Ly, i < Lx,    all k                 (Lx * Ez)
+//   Z  : k = Lz, i < Lx,    0 < j < Ly            (Lx * (Ly - 1))
+template <int Q>
+PF_DEV long long face_count(int nx, int ny, int nz) {
+    long long Lx = Ext<Q>::Lx(nx), Ly = Ext<Q>::Ly(ny), Lz = Ext<Q>::Lz(nz);
+    return (Ly + 1) * (Lz + 1) + 2 * Lx * (Lz + 1) + Lx * (Ly - 1);
+}
+
+template <int Q>
+PF_DEV void face_cell(int nx, int ny, int nz, long long t, int& i, int& j, int& k) {
+    const long long Lx = Ext<Q>::Lx(nx), Ly = Ext<Q>::Ly(ny), Lz = Ext<Q>::Lz(nz);
+    const long long Ez = Lz + 1;
+    long long nX = (Ly + 1) * Ez;
+    if (t < nX) { i = (int)Lx; j = (int)(t / Ez); k = (int)(t % Ez); return; }
+    t -= nX;
+    long long nY = Lx * Ez;
+    if (t < nY) { j = 0; i = (int)(t / Ez); k = (int)(t % Ez); return; }
+    t -= nY;
+    if (t < nY) { j = (int)Ly; i = (int)(t / Ez); k = (int)(t % Ez); return; }
+    t -= nY;
+    k = (int)Lz;
+    i = (int)(t / (Ly - 1));
+    j = 1 + (int)(t % (Ly - 1));
+}
+
+template <int Q, int MODE, class R>
+PF_DEV void face_write(const Mesh<R>& m, long long t, R w) {
+    int i, j, k;
+    face_cell<Q>(m.nx, m.ny, m.nz, t, i, j, k);
+    V<R> v = final_value<Q, MODE>(m, i, j, k, w);
+    long long id = at(m, i, j, k);
+    R* n = Q == 0 ? m.n0 : (Q == 1 ? m.n1 : m.n2);
+    R* nl = Q == 0 ? m.nl0 : (Q == 1 ? m.nl1 : m.nl2);
+    n[id] = v.h;
+    if constexpr (MODE == MODE_FP32C) nl[id] = v.l;
+}
+
+template <class R, int MODE>
+PF_DEV void faces_one(const Mesh<R>& m, long long t) {
+    const long long tt = step_index(m);
+    if (tt >= m.nsteps) return;
+    const R w = (R)m.wave[tt];
+    const long long n0 = face_count<0>(m.nx, m.ny, m.nz);
+    const long long n1 = face_count<1>(m.nx, m.ny, m.nz);
+    const long long n2 = face_count<2>(m.nx, m.ny, m.nz);
+    if (t < n0) { face_write<0, MODE>(m, t, w); return; }
+    t -= n0;
+    if (t < n1) { face_write<1, MODE>(m, t, w); return; }
+    t -= n1;
+    if (t < n2) face_write<2, MODE>(m, t, w);
+}
+
+template <class R, int MODE>
+__global__ __launch_bounds__(256) void k_faces(Mesh<R> m) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    faces_one<R, MODE>(m, t);
+}
+
+template <class R>
+inline long long face_total(int nx, int ny, int nz) {
+    auto cnt = [&](long long Lx, long long Ly, long long Lz) {
+        return (Ly + 1) * (Lz + 1) + 2 * Lx * (Lz + 1) + Lx * (Ly - 1);
+    };
+    return cnt(nx - 2, ny - 1, nz - 1) + cnt(nx - 1, ny - 2, nz - 1) + cnt(nx - 1, ny - 1, nz - 2);
+}
+
+// ------------------------------------------------------------ probes ----
+// Surface u_z of the newest level at (i0 + si*a, j0 + sj*b, 0) into frame
+// slot (*frame % cap); one block, so the counter update is ordered after
+// every read by the block barrier.
+template <class R>
+__global__ __launch_bounds__(256) void k_probe(const R* __restrict__ uz, const R* __restrict__ uzl,
+                                               int plane_ny_pz_dummy, long long plane, int pz,
+                                               int i0, int si, int ni, int j0, int sj, int nj,
+                                               double* __restrict__ ring, int cap,
+                                               long long* __restrict__ frame,
+                                               const long long* __restrict__ step_base,
+                                               long long step_off, int every, long long nsteps) {
+    (void)plane_ny_pz_dummy;
+    long long tt = (step_base ? *step_base : 0) + step_off;  // step just completed
+    if (tt >= nsteps || (tt + 1) % every != 0) return;
+    long long f = *frame;
+    double* out = ring + (f % cap) * (long long)ni * nj;
+    for (int p = threadIdx.x; p < ni * nj; p += blockDim.x) {
+        int a = p / nj, b = p % nj;
+        long long id = (long long)(i0 + si * a) * plane + (long long)(j0 + sj * b) * pz;
+        double v = (double)uz[id];
+        if (uzl) v += (double)uzl[id];
+        out[p] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *frame = f + 1;
+}
+
+// Tiny kernel: advance a device step counter after a captured chunk.
+static __global__ void k_advance(long long* counter, long long by, long long cap) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        long long v = *counter + by;
+        *counter = v < cap ? v : cap;
+    }
+}
+
+// ------------------------------------------------------- pack/unpack ----
+// Dense reference-shaped double array <-> padded device component.
+template <class R>
+__global__ void k_pack(const R* __restrict__ hi, const R* __restrict__ lo, long long plane, int pz,
+                       int ex, int ey, int ez, double* __restrict__ out,
+                       int subst, double w) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long n = (long long)ex * ey * ez;
+    if (t >= n) return;
+    int k = (int)(t % ez);
+    long long r = t / ez;
+    int j = (int)(r % ey);
+    int i = (int)(r / ey);
+    long long id = (long long)i * plane + (long long)j * pz + k;
+    double v = (double)hi[id];
+    if (lo) v += (double)lo[id];
+    if (subst && i == 0 && k == 0) v = w;
+    out[t] = v;
+}
+
+template <class R>
+__global__ void k_unpack(R* __restrict__ hi, R* __restrict__ lo, long long plane, int pz,
+                         int ex, int ey, int ez, const double* __restrict__ in) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long n = (long long)ex * ey * ez;
+    if (t >= n) return;
+    int k = (int)(t % ez);
+    long long r = t / ez;
+    int j = (int)(r % ey);
+    int i = (int)(r / ey);
+    long long id = (long long)i * plane + (long long)j * pz + k;
+    double v = in[t];
+    R h = (R)v;
+    hi[id] = h;
+    if (lo) lo[id] = (R)(v - (double)h);
+}
+
+}  // namespace pf

@@ -1,0 +1,123 @@
+"""Host-side assembly of a device problem from a Grid, a Material and a cfg.
+
+Everything here is set-up arithmetic the reference does on the host before
+or inside its step loop, restated so the device receives the same FP64
+numbers: spacings (grid.py:112-127), dt**2 (base_solver.py:443), the Mur
+face coefficients (base_solver.py:525-537) and the source table
+(base_solver.py:251, 294-312).  Extensions: the sponge profile and the
+probe grid.
+"""
+import numpy as np
+
+from .fdtd import MeshSpec
+
+# Faces that carry the Mur condition in the reference (and the sponge here):
+# x = max, y = 0, y = max, z = max.  No absorber at x = 0 (source wall) or
+# z = 0 (free surface).
+SPONGE_FACES = {"x": (False, True), "y": (True, True), "z": (False, True)}
+DEFAULT_SPONGE_STRENGTH = 0.15
+
+
+def mur_coefficients(row, dt, fd, sd):
+    """12 Mur coefficients from the material at node (0,0,0).
+
+    ``row`` is that node's 13-entry material row; ``fd``/``sd`` the 1-D
+    spacing arrays per axis.  Order: faces x=max, y=0, y=max, z=max, each
+    (ux, uy, uz); longitudinal speed for the face-normal component,
+    transverse for the others (base_solver.py:525-554).
+    """
+    c11, c44, p = row[0], row[9], row[12]
+    vl = np.sqrt(c11 / p)
+    vt = np.sqrt(c44 / p)
+    fdx, fdy, fdz = fd
+    sdx, sdy, sdz = sd
+    ctx = (vt * dt - fdx) / (vt * dt + fdx)
+    clx = (vl * dt - sdx) / (vl * dt + sdx)
+    cty = (vt * dt - fdy) / (vt * dt + fdy)
+    cly = (vl * dt - sdy) / (vl * dt + sdy)
+    ctz = (vt * dt - fdz) / (vt * dt + fdz)
+    clz = (vl * dt - sdz) / (vl * dt + sdz)
+    return np.array([clx[-1], ctx[-1], ctx[-1],
+                     cty[0], cly[0], cty[0],
+                     cty[-1], cly[-1], cty[-1],
+                     ctz[-1], ctz[-1], clz[-1]], dtype=np.float64)
+
+
+def sponge_axis(n, width, strength, lo, hi):
+    """Damping term s = sigma*dt/2 per node along one axis (extension).
+
+    s(d) = strength * ((width - d) / width)^2 for a node d < width cells from
+    a damped face; the step becomes
+    u_new = ((2u - (1-s) u_old) + coef*div) / (1 + s).
+    """
+    s = np.zeros(n, dtype=np.float64)
+    if width <= 0:
+        return s
+    for i in range(n):
+        v = 0.0
+        if lo and i < width:
+            r = (width - i) / width
+            v += strength * r * r
+        if hi and (n - 1 - i) < width:
+            r = (width - (n - 1 - i)) / width
+            v += strength * r * r
+        s[i] = v
+    return s
+
+
+def sponge_tables(nodes, sponge):
+    """Per-axis damping tables from cfg {'width': cells, 'strength': s}."""
+    if not sponge:
+        return None
+    width = int(sponge.get("width", 0))
+    strength = float(sponge.get("strength", DEFAULT_SPONGE_STRENGTH))
+    return [sponge_axis(n, width, strength, *SPONGE_FACES[a]) for n, a in zip(nodes, "xyz")]
+
+
+def axis_spacings(g):
+    """1-D full and staggered spacings of a Grid (grid.py:117-127)."""
+    fd = [np.ascontiguousarray(a.ravel()) for a in (g.fdx, g.fdy, g.fdz)]
+    sd = [np.ascontiguousarray(a.ravel()) for a in (g.sdx, g.sdy, g.sdz)]
+    return fd, sd
+
+
+def probe_grid(spec, nodes):
+    """Normalise a probe spec to (i0, si, ni, j0, sj, nj) on the u_z surface.
+
+    ``spec`` is None, a 6-sequence, or {'stride': s} / {'count': n} for an
+    evenly strided grid covering the surface.
+    """
+    if spec is None:
+        return None
+    nx, ny, _ = nodes
+    if isinstance(spec, dict):
+        if "count" in spec:
+            n = int(spec["count"])
+            si, sj = max(1, nx // n), max(1, ny // n)
+            ni, nj = min(n, (nx - 1) // si + 1), min(n, (ny - 1) // sj + 1)
+        else:
+            s = int(spec.get("stride", 1))
+            si = sj = s
+            ni, nj = (nx - 1) // s + 1, (ny - 1) // s + 1
+        return (0, si, ni, 0, sj, nj)
+    p = tuple(int(v) for v in spec)
+    if len(p) != 6:
+        raise ValueError("probe must be (i0, si, ni, j0, sj, nj)")
+    return p
+
+
+def mesh_spec(g, m, wave, cfg):
+    """MeshSpec of a solver's grid/material/source under its cfg."""
+    nodes = (g.x.size, g.y.size, g.z.size)
+    fd, sd = axis_spacings(g)
+    rows = m.rows()
+    dt = m.dt
+    dt2 = dt ** 2
+    mur = mur_coefficients(rows[int(m.ids[0, 0, 0])], dt, fd, sd)
+    probe = probe_grid(cfg.get("probe"), nodes)
+    every = int(cfg.get("record_every", 10))
+    steps = len(wave)
+    cap = int(cfg.get("probe_capacity", 0)) or max(1, steps // max(every, 1))
+    return MeshSpec(nodes, fd, sd, rows, m.ids, dt, dt2, mur, wave,
+                    sponge=sponge_tables(nodes, cfg.get("sponge")),
+                    probe=probe, record_every=every, probe_capacity=cap)

@@ -1,0 +1,111 @@
+"""GPU solver plugin: phonomena's step loop on an MI355X.
+
+A drop-in ``Solver`` for the reference's plugin discovery
+(phonomena/common.py:88-110): no-argument constructor, ``name``,
+``description``, ``cfg``; ``init(grid, material, steps)``;
+``run(*args, signals=...)`` blocking, emitting ``status``/``progress`` and
+leaving the final fields in ``self.g`` (as base_solver.py:224-280 does);
+``cancel``; ``test``.  The stencil runs in libphonofdtd through
+``phonomena_amd.fdtd``; there is no CPU path.
+
+cfg keys added to the reference's {wave, wave_args, write_mode}:
+  precision     'fp64' (reference arithmetic) | 'fp32' | 'fp32c' (compensated)
+  kernel        'auto' | 'naive' | 'tiled'
+  chunk         steps issued between cancel checks / progress updates
+  probe         None | [i0, si, ni, j0, sj, nj] | {'stride': s} | {'count': n}
+  record_every  probe frame every k steps (frames land in ``self.probes``)
+  sponge        None | {'width': cells, 'strength': s}  (extension absorber)
+"""
+import logging
+from time import time
+
+import numpy as np
+
+from .. import base_solver
+from ...gui.worker import WorkerSignals
+from ... import fdtd, problem
+
+logger = logging.getLogger(__name__)
+
+cfg = {
+    "precision": "fp64",
+    "kernel": "auto",
+    "chunk": 50,
+    "probe": None,
+    "record_every": 10,
+    "sponge": None,
+}
+
+
+class Solver(base_solver.BaseSolver):
+
+    def __init__(self):
+        super().__init__(logger)
+        self.name = "hip"
+        self.description = ("<p>MI355X (gfx950) HIP solver: fused stress/displacement sweeps, "
+                            "FP64 reference arithmetic or FP32 production modes.</p>")
+        self.cfg = {**self.cfg, **cfg}
+        self.ctx = None
+        self.probes = None
+        self.elapsed = None
+
+    def init(self, grid, material, steps):
+        super().init(grid, material, steps)
+        if self.ctx is not None:
+            self.ctx.close()
+            self.ctx = None
+        spec = problem.mesh_spec(self.g, self.m, self.source_table(self.t), self.cfg)
+        self.ctx = fdtd.Context(spec, precision=self.cfg["precision"], kernel=self.cfg["kernel"])
+        self.probes = None
+
+    def run(self, *args, **kwargs):
+        signals = kwargs.get("signals") or WorkerSignals()
+        if self.ctx is None:
+            raise RuntimeError("Solver.run() before init()")
+        writing = self.cfg["write_mode"] != "off"
+        chunk = 1 if writing else max(1, int(self.cfg.get("chunk", 50)))
+        signals.status.emit("Solver starting..")
+        self.running.set()
+        signals.status.emit("Running simulation.")
+        stime = time()
+        progress = 0
+        signals.progress.emit(0)
+        done = self.ctx.steps_done
+        while done < self.t:
+            if not self.running.is_set():
+                self.logger.warning("Simulation cancelled.")
+                break
+            n = min(chunk, self.t - done)
+            self.ctx.step(n)
+            self.ctx.sync()
+            done += n
+            if writing:
+                self._pull_fields()
+                self.writer.put((self.g.freezeData(), done - 1))
+            if progress < 99:
+                progress = min(99, int((done / self.t) * 100))
+                signals.progress.emit(progress)
+        self.ctx.sync()
+        self.running.clear()
+        self._pull_fields(old=True)
+        self.probes = self.ctx.probes()
+        if writing:
+            self.writer.notify_finished()
+            self.writer.join(timeout=5 * 60)
+        self.elapsed = time() - stime
+        signals.status.emit("Simulation finished in {:.2f}s.".format(self.elapsed))
+        signals.progress.emit(100)
+
+    def _pull_fields(self, old=False):
+        for c in ("ux", "uy", "uz"):
+            getattr(self.g, c)[...] = self.ctx.get(c)
+            if old:
+                # after the reference's time_step, u_new == u and u_old is
+                # the previous level
+                getattr(self.g, c + "_new")[...] = getattr(self.g, c)
+                getattr(self.g, c + "_old")[...] = self.ctx.get(c + "_old")
+
+    def close(self):
+        if self.ctx is not None:
+            self.ctx.close()
+            self.ctx = None

@@ -1,0 +1,114 @@
+"""Shared test helpers: golden fixtures as oracle problems and device specs."""
+import json
+import os
+
+import numpy as np
+
+from oracle import reference_math as rm
+from oracle import brute
+from phonomena_amd import problem
+from phonomena_amd.fdtd import MeshSpec
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CASES = ["testdefaults_10", "homog16_200", "default_json_100", "nonuniform_json_60", "fine_json_20"]
+
+
+def golden(name):
+    return np.load(os.path.join(GOLDEN, name + ".npz"))
+
+
+def record_every(name):
+    return 1 if name == "testdefaults_10" else 10
+
+
+def oracle_problem(d, wave=None, sponge=None):
+    """Oracle Problem (dense reference layout) for a fixture's inputs."""
+    cfg = json.loads(str(d["cfg"]))
+    C, P = rm.dense_material(d["ids"], [d["C_primary"], d["C_secondary"]],
+                             [d["p_primary"], d["p_secondary"]])
+    dt = d["dt"][()]
+    steps = int(d["steps"])
+    if wave is None:
+        wave = rm.source_table(cfg["wave"], steps, dt, cfg["wave_args"])
+    return rm.Problem(d["x"], d["y"], d["z"], C, P, dt, wave, sponge=sponge)
+
+
+def rows_of(d):
+    return brute.material_table([d["C_primary"], d["C_secondary"]], [d["p_primary"], d["p_secondary"]])
+
+
+def spec_from(x, y, z, ids, rows, dt, wave, probe=None, every=1, sponge=None, cap=None):
+    """Device MeshSpec built the way the product builds it (problem.py)."""
+    fdx, fdy, fdz = np.diff(x), np.diff(y), np.diff(z)
+    sd = [np.mean([f[1:], f[:-1]], axis=0) for f in (fdx, fdy, fdz)]
+    fd = [fdx, fdy, fdz]
+    dt = np.float64(dt)
+    mur = problem.mur_coefficients(rows[int(ids[0, 0, 0])], dt, fd, sd)
+    nodes = (x.size, y.size, z.size)
+    if cap is None:
+        cap = max(1, len(wave) // max(every, 1))
+    return MeshSpec(nodes, fd, sd, rows, ids, dt, dt ** 2, mur, wave, sponge=sponge,
+                    probe=probe, record_every=every, probe_capacity=cap)
+
+
+def spec_from_golden(d, wave=None, probe="full", sponge=None):
+    cfg = json.loads(str(d["cfg"]))
+    dt = d["dt"][()]
+    steps = int(d["steps"])
+    if wave is None:
+        wave = rm.source_table(cfg["wave"], steps, dt, cfg["wave_args"])
+    x, y = d["x"], d["y"]
+    pr = (0, 1, x.size, 0, 1, y.size) if probe == "full" else probe
+    return spec_from(x, y, d["z"], d["ids"], rows_of(d), dt, wave, probe=pr,
+                     every=1 if steps <= 10 else 10, sponge=sponge)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    n = np.linalg.norm(b)
+    if n == 0:
+        return float(np.linalg.norm(a))
+    return float(np.linalg.norm(a - b) / n)
+
+
+def uniform_crystal(cells, d, rho=2700.0, lam=5.1e10, mu=2.6e10, incl=None, steps=100,
+                    wave_kind="sin", f=None, courant=0.5, seed=0):
+    """Uniform-spacing problem (x, y, z, ids, rows, dt, wave) for tests.
+
+    incl: None or (rho, lam, mu, a_cells, r_cells) square lattice of full-depth
+    cylinders with +-0.5 cell centre jitter drawn from ``seed``.
+    """
+    nx, ny, nz = (c + 1 for c in cells)
+    x = np.arange(nx) * d
+    y = np.arange(ny) * d
+    z = np.arange(nz) * d
+    mats = [(rho, lam, mu)]
+    ids = np.zeros((nx, ny, nz), dtype=np.uint8)
+    if incl is not None:
+        r2, l2, m2, a, r = incl
+        mats.append((r2, l2, m2))
+        rng = np.random.default_rng(seed)
+        X, Y = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+        for cx in np.arange(a / 2, cells[0], a):
+            for cy in np.arange(a / 2, cells[1], a):
+                jx, jy = rng.uniform(-0.5, 0.5, 2)
+                px, py = cx + jx, cy + jy
+                if px - r <= 0 or px + r >= cells[0] or py - r <= 0 or py + r >= cells[1]:
+                    continue
+                ids[(X - px) ** 2 + (Y - py) ** 2 < r * r, :] = 1
+    rows = []
+    for (p, l, m) in mats:
+        c = np.zeros((6, 6))
+        c[:3, :3] = l
+        for a_ in range(3):
+            c[a_, a_] = l + 2 * m
+            c[3 + a_, 3 + a_] = m
+        rows.append(c)
+    table = brute.material_table(rows, [m[0] for m in mats])
+    vmax = max(np.sqrt(t[0] / t[12]) for t in table)
+    dt = np.float64(courant * d / vmax)
+    if f is None:
+        f = 1.0 / (40 * dt)
+    wave = rm.source_table(wave_kind, steps, dt, {"f": f})
+    return x, y, z, ids, table, dt, wave
