@@ -526,6 +526,9 @@ extern "C" int pf_energy(pf_ctx* c, double* out) {
     m.sx0 = c->s0[0]; m.sy0 = c->s0[1]; m.sz0 = c->s0[2];
     ev.rho = (const double*)c->rho64;
     ev.dt = c->dt;
+    const long long done = c->step;
+    ev.subst_prev = done > 0 ? 1 : 0;
+    if (done > 0) HIPCHK(hipMemcpy(&ev.w_prev, c->wave + (done - 1), sizeof(double), hipMemcpyDeviceToHost));
     hipLaunchKernelGGL(k_energy_partial, dim3(c->e_blocks), dim3(256), 0, s, ev, c->e_partials);
     hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, s, (const double*)c->e_partials, c->e_blocks, c->e_out);
     HIPCHK(hipGetLastError());

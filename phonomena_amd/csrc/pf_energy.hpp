@@ -14,6 +14,8 @@ struct EnergyView {
     Mesh<double> m;            // c* = newest level, n* = previous level (read only)
     const double* rho;         // density per material
     double dt;
+    int subst_prev;            // previous level's uz wall line holds the source
+    double w_prev;             // value of the step that produced the newest level
 };
 
 PF_DEV double warp_sum(double v) {
@@ -23,16 +25,21 @@ PF_DEV double warp_sum(double v) {
 
 // Leapfrog-invariant potential term at one node site: the stress of the
 // newest level (c* arrays) times the strain bracket of the previous level
-// (n* arrays), over the reference's index ranges.  Raw stored values are
-// used: the source line is not substituted.
-PF_DEV double site_energy(const Mesh<double>& m, int i, int j, int k) {
+// (n* arrays), over the reference's index ranges.  As in the reference's
+// arrays after time_step, the previous level carries the source value on
+// its uz wall line (u_old was copied after the source write) and the newest
+// level carries what the step stored there.
+PF_DEV double site_energy(const EnergyView& ev, int i, int j, int k) {
+    const Mesh<double>& m = ev.m;
     double acc = 0.0;
     auto C0 = [&](int a, int b, int c) { return m.c0[at(m, a, b, c)]; };
     auto C1 = [&](int a, int b, int c) { return m.c1[at(m, a, b, c)]; };
     auto C2 = [&](int a, int b, int c) { return m.c2[at(m, a, b, c)]; };
     auto N0 = [&](int a, int b, int c) { return (double)m.n0[at(m, a, b, c)]; };
     auto N1 = [&](int a, int b, int c) { return (double)m.n1[at(m, a, b, c)]; };
-    auto N2 = [&](int a, int b, int c) { return (double)m.n2[at(m, a, b, c)]; };
+    auto N2 = [&](int a, int b, int c) {
+        return (ev.subst_prev && a == 0 && c == 0) ? ev.w_prev : (double)m.n2[at(m, a, b, c)];
+    };
     // normal sites
     if (i >= 1 && i <= m.nx - 2 && j >= 1 && j <= m.ny - 2 && k <= m.nz - 2) {
         const double* c = matp(m, i, j, k);
@@ -129,7 +136,7 @@ static __global__ __launch_bounds__(256) void k_energy_partial(EnergyView ev, do
         int j = (int)(r % m.ny);
         int i = (int)(r / m.ny);
         ke += node_kinetic(ev, i, j, k);
-        se += site_energy(m, i, j, k);
+        se += site_energy(ev, i, j, k);
     }
     ke = warp_sum(ke);
     se = warp_sum(se);
