@@ -5,8 +5,9 @@
 Workload "plain" (the headline): the plain FP32 sweep on the 512x512x256-cell
 phononic crystal (513x513x257 nodes; square lattice a = 24 cells, r = 8
 cells, +-0.5 cell jitter, seed 3; matrix rho 2700, lambda 5.1e10, mu 2.6e10;
-tungsten-like inclusions rho 19300, lambda 2.0e11, mu 1.6e11; dx = 5e-5 m;
-Courant 0.5; Gaussian-modulated 2 MHz wall source).  A step is one time step
+tungsten-like inclusions rho 19300, lambda 2.0e11, mu 1.6e11; Courant 0.5;
+Gaussian-modulated 2 MHz wall source for dx = 5e-5 m, i.e. 100 cycles per
+unit time in the reference's unit-cell coordinates).  A step is one time step
 over the whole mesh; cell-updates = nodes x steps (the reference counts
 nodes, tests/test_speed.py:52).  Fields start at zero and the source drives
 them; inputs are synthetic and resident in HBM before the timed region.
@@ -32,15 +33,18 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_NODE = {"fp32": 9 * 4 + 1, "fp32c": 18 * 4 + 1, "fp64": 9 * 8 + 1}
 
 
-def crystal_spec(cells, a, r, seed, steps, precision_kind="gauss", f0=2e6, d=5e-5, probe=None):
+def crystal_spec(cells, a, r, seed, steps, kind="gauss", f0=2e6, dx_phys=5e-5, probe=None,
+                 inclusion=None):
+    """Crystal in the reference's unit-cell coordinates; the physical cell
+    size dx_phys maps the source frequency to f0 * dx_phys per unit time."""
     from phonomena_amd import problem
     from phonomena_amd.simulation import lattice
     from phonomena_amd.simulation.base_solver import BaseSolver
 
-    g, m = lattice.crystal(cells, d, a, r, seed)
+    g, m = lattice.crystal(cells, a, r, seed, inclusion=inclusion or lattice.TUNGSTEN)
     b = BaseSolver(None)
     b.m = m
-    b.cfg["wave"], b.cfg["wave_args"] = precision_kind, {"f": f0}
+    b.cfg["wave"], b.cfg["wave_args"] = kind, {"f": f0 * dx_phys}
     wave = b.source_table(steps)
     spec = problem.mesh_spec(g, m, wave, {"probe": probe, "record_every": 10})
     return spec, g, m
@@ -87,7 +91,8 @@ def bench_plain(args):
     steps = args.warmup + args.steps
     spec, g, m = crystal_spec((512, 512, 256), 24, 8, 3, steps)
     nodes = spec.cells
-    ctx = fdtd.Context(spec, precision=args.precision, kernel=args.kernel)
+    ctx = fdtd.Context(spec, precision=args.precision, kernel=args.kernel, tile_j=args.tile_j,
+                       tile_x=args.tile_x)
     ctx.step(args.warmup)
     ctx.sync()
     t0 = time.perf_counter()
@@ -112,7 +117,8 @@ def bench_plain(args):
         "data": "synthetic",
         "config": {"workload": "plain %s sweep, 512x512x256-cell phononic crystal (513x513x257 nodes), "
                                "lattice a=24 r=8 cells, seed 3, gauss 2 MHz source" % args.precision,
-                   "nodes": nodes, "kernel": args.kernel, "fill_fraction": float(m.ids.mean())},
+                   "nodes": nodes, "kernel": args.kernel, "tile_j": args.tile_j, "tile_x": args.tile_x,
+                   "fill_fraction": float(m.ids.mean())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_from_profiles("plain", args.precision),
@@ -130,6 +136,8 @@ def main():
     ap.add_argument("--workload", default="plain", choices=["plain"])
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32c", "fp64"])
     ap.add_argument("--kernel", default="auto", choices=["auto", "naive", "tiled"])
+    ap.add_argument("--tile-j", type=int, default=0, help="tiled sweep rows per block (4, 8, 16)")
+    ap.add_argument("--tile-x", type=int, default=0, help="tiled sweep x planes per block")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -156,8 +156,15 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
     c->plane = (long long)d->ny * c->pz;
     c->cells = (long long)d->nx * c->plane;
     c->precision = d->precision;
-    c->kernel = d->kernel == PF_KERNEL_AUTO ? PF_KERNEL_NAIVE : d->kernel;
-    c->tile_j = d->tile_j > 0 ? d->tile_j : 4;
+    c->kernel = d->kernel == PF_KERNEL_AUTO ? (d->n_materials <= PF_TILED_MAX_MATERIALS ? PF_KERNEL_TILED
+                                                                                         : PF_KERNEL_NAIVE)
+                                            : d->kernel;
+    if (c->kernel == PF_KERNEL_TILED && d->n_materials > PF_TILED_MAX_MATERIALS)
+        return fail(PF_ERR_ARG, "the tiled sweep holds at most %d materials (got %d)", PF_TILED_MAX_MATERIALS,
+                    d->n_materials);
+    c->tile_j = d->tile_j > 0 ? d->tile_j : 8;
+    if (c->tile_j != 4 && c->tile_j != 8 && c->tile_j != 16)
+        return fail(PF_ERR_ARG, "tile_j must be 4, 8 or 16 (got %d)", c->tile_j);
     c->tile_x = d->tile_x > 0 ? d->tile_x : 0;
     c->esz = d->precision == PF_FP64 ? 8 : 4;
     c->sponge = d->sponge_x != nullptr;
@@ -293,6 +300,7 @@ Mesh<R> pf::make_view(const pf_ctx* c, int cur) {
     m.step_base = nullptr;
     m.step_off = 0;
     m.nsteps = c->wave_len;
+    m.nmat = c->nmat;
     return m;
 }
 
@@ -302,9 +310,7 @@ template Mesh<double> pf::make_view<double>(const pf_ctx*, int);
 // Launch the sweep of one step of mesh view m on stream s.
 template <class R, int MODE>
 static int launch_sweep_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
-    if (c->kernel == PF_KERNEL_TILED && MODE != MODE_FP64) {
-        return launch_tiled<R, MODE>(c, m, s);
-    }
+    if (c->kernel == PF_KERNEL_TILED) return launch_tiled<R, MODE>(c, m, s);
     dim3 blk(64, 4, 1);
     dim3 grd((unsigned)((c->nz + 63) / 64), (unsigned)((c->ny + 3) / 4), (unsigned)c->nx);
     if (c->sponge) hipLaunchKernelGGL((k_sweep_naive<R, MODE, true>), grd, blk, 0, s, m);
@@ -315,8 +321,8 @@ static int launch_sweep_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
 
 template <class R, int MODE>
 static int launch_faces_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
-    long long n = face_total<R>(c->nx, c->ny, c->nz);
-    unsigned g = (unsigned)((n + 255) / 256);
+    const long long n = face_total(c->nx, c->ny, c->nz);
+    const unsigned g = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL((k_faces<R, MODE>), dim3(g), dim3(256), 0, s, m);
     HIPCHK(hipGetLastError());
     return PF_OK;

@@ -20,7 +20,8 @@
 // value as a chain  face(... face(stageI(inner cell)))  of at most four maps,
 // and every stage-I value a chain reads lies strictly inside all faces, where
 // the sweep kernel has already written it.  So the sweep writes all non-face
-// cells, then one small kernel writes all face cells.  Cells of uy/uz on the
+// cells plus the z = max face cells (a single map of the cell below), and a
+// second, small kernel writes the other face cells.  Cells of uy/uz on the
 // x = 0 wall that the reference never updates are written as 0, which is
 // what the reference holds there (the source line included, whose stored
 // value after a step is the never-written 0 of u_new).
@@ -67,6 +68,7 @@ struct Mesh {
     const long long* __restrict__ step_base;  // device step counter or null
     long long step_off;                       // tt = *step_base + step_off
     long long nsteps;                         // step budget (batch members)
+    int nmat;                                 // rows in mat
 };
 
 template <class R>
@@ -211,65 +213,6 @@ PF_DEV void stage_uz(const Mesh<R>& m, int i, int j, int k, R w, R& coef, R& dv)
     }
 }
 
-// Write the stage-I result of one component at a non-face cell.
-template <int MODE, bool SPONGE, class R>
-PF_DEV void store_leap(const Mesh<R>& m, R* nh, R* nl, const R* ch, const R* cl, long long id,
-                       R coef, R dv, R s) {
-    if constexpr (MODE == MODE_FP32C) {
-        float f = coef * dv;
-        float rh, rl;
-        leap_comp<SPONGE>(ch[id], cl[id], nh[id], nl[id], f, s, rh, rl);
-        nh[id] = rh;
-        nl[id] = rl;
-    } else {
-        nh[id] = leap<MODE == MODE_FP64, SPONGE>(ch[id], nh[id], coef, dv, s);
-    }
-}
-
-// ------------------------------------------------------- naive sweep ----
-// One thread per node; every stress a stencil needs is recomputed from u.
-template <class R, int MODE, bool SPONGE>
-__global__ __launch_bounds__(256) void k_sweep_naive(Mesh<R> m) {
-    constexpr bool EXACT = MODE == MODE_FP64;
-    const int k = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    const int i = blockIdx.z;
-    if (k >= m.nz || j >= m.ny || i >= m.nx) return;
-    const long long tt = step_index(m);
-    if (tt >= m.nsteps) return;
-    const R w = (R)m.wave[tt];
-    const long long id = at(m, i, j, k);
-    const R s = SPONGE ? damp_at(m, i, j, k) : R(0);
-    // ux: non-face cells i <= nx-3, 1 <= j <= ny-2, k <= nz-2
-    if (i <= m.nx - 3 && j >= 1 && j <= m.ny - 2 && k <= m.nz - 2) {
-        R coef, dv;
-        stage_ux<EXACT>(m, i, j, k, w, coef, dv);
-        store_leap<MODE, SPONGE>(m, m.n0, m.nl0, m.c0, m.cl0, id, coef, dv, s);
-    }
-    // uy: non-face cells i <= nx-2, 1 <= j <= ny-3, k <= nz-2; wall i = 0 -> 0
-    if (i <= m.nx - 2 && j >= 1 && j <= m.ny - 3 && k <= m.nz - 2) {
-        if (i == 0) {
-            m.n1[id] = R(0);
-            if constexpr (MODE == MODE_FP32C) m.nl1[id] = R(0);
-        } else {
-            R coef, dv;
-            stage_uy<EXACT>(m, i, j, k, w, coef, dv);
-            store_leap<MODE, SPONGE>(m, m.n1, m.nl1, m.c1, m.cl1, id, coef, dv, s);
-        }
-    }
-    // uz: non-face cells i <= nx-2, 1 <= j <= ny-2, k <= nz-3; wall i = 0 -> 0
-    if (i <= m.nx - 2 && j >= 1 && j <= m.ny - 2 && k <= m.nz - 3) {
-        if (i == 0) {
-            m.n2[id] = R(0);
-            if constexpr (MODE == MODE_FP32C) m.nl2[id] = R(0);
-        } else {
-            R coef, dv;
-            stage_uz<EXACT>(m, i, j, k, w, coef, dv);
-            store_leap<MODE, SPONGE>(m, m.n2, m.nl2, m.c2, m.cl2, id, coef, dv, s);
-        }
-    }
-}
-
 // ----------------------------------------------------------- faces ----
 // A value in flight through the face chain: hi (+ lo in compensated mode).
 template <class R> struct V { R h, l; };
@@ -346,44 +289,71 @@ PF_DEV V<R> final_value(const Mesh<R>& m, int i, int j, int k, R w) {
     return after_y1<Q, MODE>(m, i, j, k, w);
 }
 
-// Face cells of component Q, enumerated once each:
-//   X  : i = Lx,            all j, all k          (Ey * Ez)
-//   Y0 : j = 0,  i < Lx,    all k                 (Lx * Ez)
-//   Y1 : j = Ly, i < Lx,    all k                 (Lx * Ez)
-//   Z  : k = Lz, i < Lx,    0 < j < Ly            (Lx * (Ly - 1))
+template <int Q, class R> PF_DEV R* nptr(const Mesh<R>& m) { return Q == 0 ? m.n0 : (Q == 1 ? m.n1 : m.n2); }
+template <int Q, class R> PF_DEV R* nlptr(const Mesh<R>& m) { return Q == 0 ? m.nl0 : (Q == 1 ? m.nl1 : m.nl2); }
+
+template <int Q, int MODE, class R>
+PF_DEV void put(const Mesh<R>& m, int i, int j, int k, V<R> v) {
+    const long long id = at(m, i, j, k);
+    nptr<Q>(m)[id] = v.h;
+    if constexpr (MODE == MODE_FP32C) nlptr<Q>(m)[id] = v.l;
+}
+
+// Write the stage-I value s0 of a non-face cell, and, when k = Lz-1, the
+// z = max face cell above it, whose final value is a single Mur map of s0
+// (there the data is already in the sweep's registers; a separate pass
+// would read it back with a stride of one row per cell).  `cur` is the
+// cell's own level-n value.
+template <int Q, int MODE, class R>
+PF_DEV void put_with_zface(const Mesh<R>& m, int i, int j, int k, V<R> s0, V<R> cur, R w) {
+    put<Q, MODE>(m, i, j, k, s0);
+    const int Lz = Ext<Q>::Lz(m.nz);
+    if (k == Lz - 1)
+        put<Q, MODE>(m, i, j, Lz, mur_map<MODE>(cur, m.mur[9 + Q], m.mur_lo[9 + Q], s0, cur_q<Q>(m, i, j, Lz, w)));
+}
+
+// Face cells of component Q not written by the sweep, once each, z fastest
+// so each run of threads touches contiguous memory:
+//   X  : (Lx, j, k)   0 < j < Ly, k < Lz
+//   Y0 : (i, 0, k)    i < Lx,     k < Lz
+//   Y1 : (i, Ly, k)   i < Lx,     k < Lz
+//   box edges (two or more faces meet):
+//        (Lx, 0, k), (Lx, Ly, k)   all k
+//        (Lx, j, Lz)               0 < j < Ly
+//        (i, 0, Lz), (i, Ly, Lz)   i < Lx
 template <int Q>
 PF_DEV long long face_count(int nx, int ny, int nz) {
-    long long Lx = Ext<Q>::Lx(nx), Ly = Ext<Q>::Ly(ny), Lz = Ext<Q>::Lz(nz);
-    return (Ly + 1) * (Lz + 1) + 2 * Lx * (Lz + 1) + Lx * (Ly - 1);
+    const long long Lx = Ext<Q>::Lx(nx), Ly = Ext<Q>::Ly(ny), Lz = Ext<Q>::Lz(nz);
+    return (Ly - 1) * Lz + 2 * Lx * Lz + 2 * (Lz + 1) + (Ly - 1) + 2 * Lx;
 }
 
 template <int Q>
 PF_DEV void face_cell(int nx, int ny, int nz, long long t, int& i, int& j, int& k) {
     const long long Lx = Ext<Q>::Lx(nx), Ly = Ext<Q>::Ly(ny), Lz = Ext<Q>::Lz(nz);
-    const long long Ez = Lz + 1;
-    long long nX = (Ly + 1) * Ez;
-    if (t < nX) { i = (int)Lx; j = (int)(t / Ez); k = (int)(t % Ez); return; }
-    t -= nX;
-    long long nY = Lx * Ez;
-    if (t < nY) { j = 0; i = (int)(t / Ez); k = (int)(t % Ez); return; }
-    t -= nY;
-    if (t < nY) { j = (int)Ly; i = (int)(t / Ez); k = (int)(t % Ez); return; }
-    t -= nY;
-    k = (int)Lz;
-    i = (int)(t / (Ly - 1));
-    j = 1 + (int)(t % (Ly - 1));
+    long long n = (Ly - 1) * Lz;
+    if (t < n) { i = (int)Lx; j = 1 + (int)(t / Lz); k = (int)(t % Lz); return; }
+    t -= n;
+    n = Lx * Lz;
+    if (t < n) { j = 0; i = (int)(t / Lz); k = (int)(t % Lz); return; }
+    t -= n;
+    if (t < n) { j = (int)Ly; i = (int)(t / Lz); k = (int)(t % Lz); return; }
+    t -= n;
+    if (t < Lz + 1) { i = (int)Lx; j = 0; k = (int)t; return; }
+    t -= Lz + 1;
+    if (t < Lz + 1) { i = (int)Lx; j = (int)Ly; k = (int)t; return; }
+    t -= Lz + 1;
+    if (t < Ly - 1) { i = (int)Lx; j = 1 + (int)t; k = (int)Lz; return; }
+    t -= Ly - 1;
+    if (t < Lx) { i = (int)t; j = 0; k = (int)Lz; return; }
+    t -= Lx;
+    i = (int)t; j = (int)Ly; k = (int)Lz;
 }
 
 template <int Q, int MODE, class R>
 PF_DEV void face_write(const Mesh<R>& m, long long t, R w) {
     int i, j, k;
     face_cell<Q>(m.nx, m.ny, m.nz, t, i, j, k);
-    V<R> v = final_value<Q, MODE>(m, i, j, k, w);
-    long long id = at(m, i, j, k);
-    R* n = Q == 0 ? m.n0 : (Q == 1 ? m.n1 : m.n2);
-    R* nl = Q == 0 ? m.nl0 : (Q == 1 ? m.nl1 : m.nl2);
-    n[id] = v.h;
-    if constexpr (MODE == MODE_FP32C) nl[id] = v.l;
+    put<Q, MODE>(m, i, j, k, final_value<Q, MODE>(m, i, j, k, w));
 }
 
 template <class R, int MODE>
@@ -391,28 +361,91 @@ PF_DEV void faces_one(const Mesh<R>& m, long long t) {
     const long long tt = step_index(m);
     if (tt >= m.nsteps) return;
     const R w = (R)m.wave[tt];
-    const long long n0 = face_count<0>(m.nx, m.ny, m.nz);
-    const long long n1 = face_count<1>(m.nx, m.ny, m.nz);
-    const long long n2 = face_count<2>(m.nx, m.ny, m.nz);
-    if (t < n0) { face_write<0, MODE>(m, t, w); return; }
-    t -= n0;
-    if (t < n1) { face_write<1, MODE>(m, t, w); return; }
-    t -= n1;
-    if (t < n2) face_write<2, MODE>(m, t, w);
+    const long long e0 = face_count<0>(m.nx, m.ny, m.nz);
+    const long long e1 = face_count<1>(m.nx, m.ny, m.nz);
+    const long long e2 = face_count<2>(m.nx, m.ny, m.nz);
+    if (t < e0) { face_write<0, MODE>(m, t, w); return; }
+    t -= e0;
+    if (t < e1) { face_write<1, MODE>(m, t, w); return; }
+    t -= e1;
+    if (t < e2) face_write<2, MODE>(m, t, w);
 }
 
 template <class R, int MODE>
 __global__ __launch_bounds__(256) void k_faces(Mesh<R> m) {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    faces_one<R, MODE>(m, t);
+    faces_one<R, MODE>(m, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-template <class R>
 inline long long face_total(int nx, int ny, int nz) {
-    auto cnt = [&](long long Lx, long long Ly, long long Lz) {
-        return (Ly + 1) * (Lz + 1) + 2 * Lx * (Lz + 1) + Lx * (Ly - 1);
+    auto cnt = [](long long Lx, long long Ly, long long Lz) {
+        return (Ly - 1) * Lz + 2 * Lx * Lz + 2 * (Lz + 1) + (Ly - 1) + 2 * Lx;
     };
     return cnt(nx - 2, ny - 1, nz - 1) + cnt(nx - 1, ny - 2, nz - 1) + cnt(nx - 1, ny - 1, nz - 2);
+}
+
+// Stage-I value of one component at a cell: the leapfrog of its current
+// value cur (hi + lo), its previous value (oh + ol) and coef*div.
+template <int MODE, bool SPONGE, class R>
+PF_DEV V<R> leap_from(V<R> cur, R oh, R ol, R coef, R dv, R s) {
+    V<R> r;
+    if constexpr (MODE == MODE_FP32C) {
+        leap_comp<SPONGE>(cur.h, cur.l, oh, ol, coef * dv, s, r.h, r.l);
+    } else {
+        r.h = leap<MODE == MODE_FP64, SPONGE>(cur.h, oh, coef, dv, s);
+        r.l = R(0);
+    }
+    return r;
+}
+
+template <int MODE, bool SPONGE, class R>
+PF_DEV V<R> leap_value(V<R> cur, const R* nh, const R* nl, long long id, R coef, R dv, R s) {
+    return leap_from<MODE, SPONGE>(cur, nh[id], nl ? nl[id] : R(0), coef, dv, s);
+}
+
+// ------------------------------------------------------- naive sweep ----
+// One thread per node; every stress a stencil needs is recomputed from u.
+// Writes the non-face cells and the z = max face cells above them.
+template <class R, int MODE, bool SPONGE>
+__global__ __launch_bounds__(256) void k_sweep_naive(Mesh<R> m) {
+    constexpr bool EXACT = MODE == MODE_FP64;
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k >= m.nz || j >= m.ny || i >= m.nx) return;
+    const long long tt = step_index(m);
+    if (tt >= m.nsteps) return;
+    const R w = (R)m.wave[tt];
+    const long long id = at(m, i, j, k);
+    const R s = SPONGE ? damp_at(m, i, j, k) : R(0);
+    // ux: non-face cells i <= nx-3, 1 <= j <= ny-2, k <= nz-2
+    if (i <= m.nx - 3 && j >= 1 && j <= m.ny - 2 && k <= m.nz - 2) {
+        R coef, dv;
+        stage_ux<EXACT>(m, i, j, k, w, coef, dv);
+        V<R> cur = cur_q<0>(m, i, j, k, w);
+        put_with_zface<0, MODE>(m, i, j, k, leap_value<MODE, SPONGE>(cur, m.n0, m.nl0, id, coef, dv, s), cur, w);
+    }
+    // uy: non-face cells i <= nx-2, 1 <= j <= ny-3, k <= nz-2; wall i = 0 -> 0
+    if (i <= m.nx - 2 && j >= 1 && j <= m.ny - 3 && k <= m.nz - 2) {
+        V<R> cur = cur_q<1>(m, i, j, k, w);
+        V<R> s0 = {R(0), R(0)};
+        if (i > 0) {
+            R coef, dv;
+            stage_uy<EXACT>(m, i, j, k, w, coef, dv);
+            s0 = leap_value<MODE, SPONGE>(cur, m.n1, m.nl1, id, coef, dv, s);
+        }
+        put_with_zface<1, MODE>(m, i, j, k, s0, cur, w);
+    }
+    // uz: non-face cells i <= nx-2, 1 <= j <= ny-2, k <= nz-3; wall i = 0 -> 0
+    if (i <= m.nx - 2 && j >= 1 && j <= m.ny - 2 && k <= m.nz - 3) {
+        V<R> cur = cur_q<2>(m, i, j, k, w);
+        V<R> s0 = {R(0), R(0)};
+        if (i > 0) {
+            R coef, dv;
+            stage_uz<EXACT>(m, i, j, k, w, coef, dv);
+            s0 = leap_value<MODE, SPONGE>(cur, m.n2, m.nl2, id, coef, dv, s);
+        }
+        put_with_zface<2, MODE>(m, i, j, k, s0, cur, w);
+    }
 }
 
 // ------------------------------------------------------------ probes ----

@@ -5,6 +5,9 @@
 #include "../../include/phonofdtd.h"
 #include "pf_kernels.hpp"
 
+// Material rows the tiled sweep stages in LDS.
+#define PF_TILED_MAX_MATERIALS 16
+
 // One mesh on one device.  Two levels of each displacement component are
 // kept (ping-pong); `cur` indexes the current level.
 struct pf_ctx {

@@ -34,9 +34,48 @@ def lattice_sites(cells, a, r, seed, jitter=0.5):
     return sites
 
 
-def crystal(cells, d, a, r, seed, inclusion=TUNGSTEN, matrix=MATRIX, jitter=0.5, courant=0.5):
-    """(Grid, Material) of a crystal with ``cells`` = (Nx, Ny, Nz) cells of
-    size ``d`` metres; node counts are cells + 1 per axis."""
+def pairing_speed(m):
+    """Fastest speed the stencil can realise: sqrt(C11 of one material /
+    rho of the other).  The reference samples stiffness and density at
+    different nodes without averaging (base_solver.py:353-367, 443-459), so
+    next to an interface a node can combine the stiffness of one material
+    with the density of the other."""
+    mats = [m.primary, m.secondary]
+    return max(np.sqrt(np.asarray(a['c'])[0, 0] / float(b['p'])) for a in mats for b in mats)
+
+
+def reference_speed(m):
+    """vmax of the reference's CFL rule (material.py:80-93)."""
+    return max(max(np.sqrt(np.asarray(x['c'])[0, 0] / float(x['p'])),
+                   np.sqrt(np.asarray(x['c'])[3, 3] / float(x['p']))) for x in (m.primary, m.secondary))
+
+
+def stable_courant(m, courant=0.5, ratio=4.0):
+    """Courant number for Material.c_max.
+
+    The reference's rule dt = c_max * dx_min / vmax is kept.  When the
+    stiffness/density pairing of the two materials is more than ``ratio``
+    times faster than vmax (air-like voids in a solid: ~47x), c_max is scaled
+    so that dt = courant * dx_min / pairing_speed; with the reference's
+    c_max = 0.5 such crystals diverge within a few hundred steps.
+    """
+    v_ref, v_mix = reference_speed(m), pairing_speed(m)
+    if v_mix > ratio * v_ref:
+        return courant * v_ref / v_mix
+    return courant
+
+
+def crystal(cells, a, r, seed, inclusion=TUNGSTEN, matrix=MATRIX, jitter=0.5, courant=0.5, d=1.0):
+    """(Grid, Material) of a crystal of ``cells`` = (Nx, Ny, Nz) cells.
+
+    Coordinates are in cell units by default (d = 1), the way the reference's
+    own settings are written (SI_conversion = 1, one unit per cell): the
+    scheme then runs exactly as the reference runs it.  A physical cell size
+    dx maps onto time: t_phys = t * dx and f_phys = f / dx.  (At physical
+    coordinates such as dx = 5e-5 the reference's free-surface u_z update,
+    which adds T3[..,1] without dividing by dz (base_solver.py:516), makes
+    the scheme diverge; at unit spacing that term is what was intended.)
+    """
     g = _grid.Grid()
     g.setSpacing(d)
     g.init(*cells)
@@ -46,7 +85,7 @@ def crystal(cells, d, a, r, seed, inclusion=TUNGSTEN, matrix=MATRIX, jitter=0.5,
     m.init(g, {"matrix": _material.isotropic(*matrix), "inclusion": _material.isotropic(*inclusion)})
     m.setPrimary("matrix")
     m.setSecondary("inclusion")
-    m.c_max = courant
+    m.c_max = stable_courant(m, courant)
     m.update()
     return g, m
 

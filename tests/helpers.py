@@ -106,7 +106,10 @@ def uniform_crystal(cells, d, rho=2700.0, lam=5.1e10, mu=2.6e10, incl=None, step
             c[3 + a_, 3 + a_] = m
         rows.append(c)
     table = brute.material_table(rows, [m[0] for m in mats])
-    vmax = max(np.sqrt(t[0] / t[12]) for t in table)
+    vmax = max(max(np.sqrt(t[0] / t[12]), np.sqrt(t[9] / t[12])) for t in table)
+    vmix = max(np.sqrt(a_[0] / b_[12]) for a_ in table for b_ in table)
+    if vmix > 4 * vmax:  # air-like voids: Courant of the fastest pairing
+        courant = courant * vmax / vmix
     dt = np.float64(courant * d / vmax)
     if f is None:
         f = 1.0 / (40 * dt)

@@ -1,0 +1,64 @@
+"""The LDS-tiled sweep against the naive sweep: same formulas on the same
+operands, so identical bits in every precision, on uniform and non-uniform
+meshes, with and without the sponge, for every tile shape."""
+import numpy as np
+import pytest
+
+from helpers import CASES, golden, spec_from, spec_from_golden, uniform_crystal
+from phonomena_amd import fdtd, problem
+
+pytestmark = pytest.mark.gpu
+FIELDS = ["ux", "uy", "uz", "ux_old", "uy_old", "uz_old"]
+
+
+def _fields(spec, precision, kernel, tile_j=0, tile_x=0, steps=None):
+    ctx = fdtd.Context(spec, precision=precision, kernel=kernel, tile_j=tile_j, tile_x=tile_x)
+    ctx.step(spec.wave.size if steps is None else steps)
+    ctx.sync()
+    out = {f: ctx.get(f) for f in FIELDS}
+    out["probes"] = ctx.probes()
+    ctx.close()
+    return out
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp32c"])
+def test_tiled_equals_naive_on_golden_meshes(name, precision):
+    spec = spec_from_golden(golden(name))
+    a = _fields(spec, precision, "naive")
+    b = _fields(spec, precision, "tiled")
+    for f in a:
+        assert np.array_equal(a[f], b[f]), f
+
+
+def test_fp64_tiled_bitwise_vs_reference():
+    for name in CASES:
+        d = golden(name)
+        out = _fields(spec_from_golden(d), "fp64", "tiled")
+        for f in FIELDS:
+            assert np.array_equal(out[f], d[f]), (name, f)
+
+
+@pytest.mark.parametrize("tile_j,tile_x", [(4, 0), (8, 7), (16, 0), (16, 33)])
+@pytest.mark.parametrize("precision", ["fp32", "fp32c"])
+def test_tiled_shapes_on_crystal(tile_j, tile_x, precision):
+    # z extent not a multiple of the 62 output lanes, y not of the row count
+    x, y, z, ids, rows, dt, wave = uniform_crystal(
+        (70, 45, 131), 1.0, incl=(19300.0, 2.0e11, 1.6e11, 14, 5), steps=120, seed=5)
+    spec = spec_from(x, y, z, ids, rows, dt, wave, probe=(0, 3, 24, 1, 2, 23), every=7)
+    a = _fields(spec, precision, "naive")
+    b = _fields(spec, precision, "tiled", tile_j=tile_j, tile_x=tile_x)
+    for f in a:
+        assert np.array_equal(a[f], b[f]), f
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp32c"])
+def test_tiled_sponge_equals_naive(precision):
+    x, y, z, ids, rows, dt, wave = uniform_crystal(
+        (40, 40, 40), 1.0, incl=(1.2, 1.4e5, 0.0, 10, 3), steps=150, seed=9)
+    sponge = problem.sponge_tables((41, 41, 41), {"width": 8, "strength": 0.15})
+    spec = spec_from(x, y, z, ids, rows, dt, wave, sponge=sponge)
+    a = _fields(spec, precision, "naive")
+    b = _fields(spec, precision, "tiled")
+    for f in a:
+        assert np.array_equal(a[f], b[f]), f

@@ -52,11 +52,12 @@ def _oracle_brute(x, y, z, ids, rows, dt, wave, steps, probe=None, every=0, spon
 
 
 def test_fp64_homogeneous_64x64x32_1000_steps():
-    """Plain reference case: 64x64x32 cells, dx = 1e-4 m, homogeneous,
-    Courant 0.5, 1 MHz sine on the x = 0 wall, 1000 steps, surface u_z
-    every 10 steps; FP64 must match the oracle to rel-L2 <= 1e-10."""
-    x, y, z, ids, rows, dt, _ = uniform_crystal((64, 64, 32), 1e-4, steps=1)
-    wave = rm.source_table("sin", 1000, dt, {"f": 1e6})
+    """Plain reference case: 64x64x32 cells, homogeneous, Courant 0.5, sine
+    on the x = 0 wall at 1 MHz for dx = 1e-4 m (100 cycles per unit time in
+    the reference's unit-cell coordinates), 1000 steps, surface u_z every 10
+    steps; FP64 must match the oracle to rel-L2 <= 1e-10."""
+    x, y, z, ids, rows, dt, _ = uniform_crystal((64, 64, 32), 1.0, steps=1)
+    wave = rm.source_table("sin", 1000, dt, {"f": 1e6 * 1e-4})
     probe = (0, 1, 65, 0, 1, 65)
     st, frames = _oracle_brute(x, y, z, ids, rows, dt, wave, 1000, probe=probe, every=10)
     ctx = _run(spec_from(x, y, z, ids, rows, dt, wave, probe=probe, every=10))
@@ -68,7 +69,7 @@ def test_fp64_homogeneous_64x64x32_1000_steps():
 
 def test_fp64_crystal_two_materials():
     x, y, z, ids, rows, dt, wave = uniform_crystal(
-        (40, 36, 20), 5e-5, incl=(19300.0, 2.0e11, 1.6e11, 12, 4), steps=300, seed=2)
+        (40, 36, 20), 1.0, incl=(19300.0, 2.0e11, 1.6e11, 12, 4), steps=300, seed=2)
     assert ids.any()
     st, _ = _oracle_brute(x, y, z, ids, rows, dt, wave, 300)
     ctx = _run(spec_from(x, y, z, ids, rows, dt, wave))
@@ -80,7 +81,7 @@ def test_fp64_crystal_two_materials():
 @pytest.mark.parametrize("precision,tol", [("fp32", 2e-4), ("fp32c", 2e-4)])
 def test_fp32_modes_vs_fp64_oracle(precision, tol):
     x, y, z, ids, rows, dt, wave = uniform_crystal(
-        (48, 40, 24), 5e-5, incl=(19300.0, 2.0e11, 1.6e11, 16, 5), steps=600, wave_kind="sin", seed=3)
+        (48, 40, 24), 1.0, incl=(19300.0, 2.0e11, 1.6e11, 16, 5), steps=600, wave_kind="sin", seed=3)
     st, _ = _oracle_brute(x, y, z, ids, rows, dt, wave, 600)
     ctx = _run(spec_from(x, y, z, ids, rows, dt, wave), precision=precision)
     errs = {f: rel_l2(ctx.get(f), st[f]) for f in ("ux", "uy", "uz")}
