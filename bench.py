@@ -135,7 +135,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="plain", choices=["plain"])
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32c", "fp64"])
-    ap.add_argument("--kernel", default="auto", choices=["auto", "naive", "tiled"])
+    ap.add_argument("--kernel", default="auto", choices=["auto", "naive", "tiled", "strip"])
     ap.add_argument("--tile-j", type=int, default=0, help="tiled sweep rows per block (4, 8, 16)")
     ap.add_argument("--tile-x", type=int, default=0, help="tiled sweep x planes per block")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")

@@ -41,6 +41,8 @@ extern "C" {
 #define PF_KERNEL_AUTO 0
 #define PF_KERNEL_NAIVE 1   /* one thread per node, stresses recomputed     */
 #define PF_KERNEL_TILED 2   /* LDS-staged (y,z) tiles, register march in x  */
+#define PF_KERNEL_STRIP 3   /* barrier-free: each wavefront owns a strip of
+                               y rows in registers, z neighbours by DPP     */
 
 /* Field selectors for pf_get_field / pf_set_field. */
 #define PF_UX 0
@@ -82,8 +84,9 @@ typedef struct pf_desc {
                                   (ni == 0: no probes)                          */
     int32_t record_every;      /* probe frame after every k-th step            */
     int32_t probe_capacity;    /* ring size in frames                          */
-    int32_t tile_j;            /* tiled kernel: rows per tile (0 = default)    */
-    int32_t tile_x;            /* tiled kernel: x planes per tile (0 = all)    */
+    int32_t tile_j;            /* rows per block (tiled: 4/8/16) or per
+                                  wavefront (strip: 4/8); 0 = default          */
+    int32_t tile_x;            /* x planes per block (0 = automatic)           */
 } pf_desc;
 
 /* Library / device identification. */

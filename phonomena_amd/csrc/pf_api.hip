@@ -13,6 +13,7 @@
 #include "pf_energy.hpp"
 #include "pf_kernels.hpp"
 #include "pf_runtime.hpp"
+#include "pf_strip.hpp"
 #include "pf_tiled.hpp"
 
 using namespace pf;
@@ -80,7 +81,7 @@ int check_desc(const pf_desc* d) {
         return fail(PF_ERR_ARG, "node counts must be >= 4 per axis (got %d x %d x %d)", d->nx, d->ny, d->nz);
     if ((long long)d->nx * d->ny * d->nz > (1LL << 40)) return fail(PF_ERR_ARG, "mesh too large");
     if (d->precision < PF_FP64 || d->precision > PF_FP32C) return fail(PF_ERR_ARG, "bad precision %d", d->precision);
-    if (d->kernel < PF_KERNEL_AUTO || d->kernel > PF_KERNEL_TILED) return fail(PF_ERR_ARG, "bad kernel %d", d->kernel);
+    if (d->kernel < PF_KERNEL_AUTO || d->kernel > PF_KERNEL_STRIP) return fail(PF_ERR_ARG, "bad kernel %d", d->kernel);
     if (d->n_materials < 1 || d->n_materials > 256) return fail(PF_ERR_ARG, "n_materials must be 1..256");
     if (!d->fdx || !d->fdy || !d->fdz || !d->sdx || !d->sdy || !d->sdz) return fail(PF_ERR_ARG, "spacing table missing");
     if (!d->materials || !d->ids) return fail(PF_ERR_ARG, "material table or ids missing");
@@ -156,15 +157,21 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
     c->plane = (long long)d->ny * c->pz;
     c->cells = (long long)d->nx * c->plane;
     c->precision = d->precision;
-    c->kernel = d->kernel == PF_KERNEL_AUTO ? (d->n_materials <= PF_TILED_MAX_MATERIALS ? PF_KERNEL_TILED
-                                                                                         : PF_KERNEL_NAIVE)
-                                            : d->kernel;
-    if (c->kernel == PF_KERNEL_TILED && d->n_materials > PF_TILED_MAX_MATERIALS)
-        return fail(PF_ERR_ARG, "the tiled sweep holds at most %d materials (got %d)", PF_TILED_MAX_MATERIALS,
-                    d->n_materials);
-    c->tile_j = d->tile_j > 0 ? d->tile_j : 8;
-    if (c->tile_j != 4 && c->tile_j != 8 && c->tile_j != 16)
-        return fail(PF_ERR_ARG, "tile_j must be 4, 8 or 16 (got %d)", c->tile_j);
+    const bool few_mats = d->n_materials <= PF_TILED_MAX_MATERIALS;
+    const bool small = (unsigned long long)c->cells * (d->precision == PF_FP64 ? 8 : 4) < (1ULL << 32);
+    c->kernel = d->kernel == PF_KERNEL_AUTO ? (few_mats && small ? PF_KERNEL_TILED : PF_KERNEL_NAIVE) : d->kernel;
+    if ((c->kernel == PF_KERNEL_TILED || c->kernel == PF_KERNEL_STRIP) && !few_mats)
+        return fail(PF_ERR_ARG, "the tiled and strip sweeps hold at most %d materials (got %d)",
+                    PF_TILED_MAX_MATERIALS, d->n_materials);
+    if (c->kernel == PF_KERNEL_STRIP) {
+        c->tile_j = d->tile_j > 0 ? d->tile_j : 4;
+        if (c->tile_j != 4 && c->tile_j != 8)
+            return fail(PF_ERR_ARG, "strip rows must be 4 or 8 (got %d)", c->tile_j);
+    } else {
+        c->tile_j = d->tile_j > 0 ? d->tile_j : (d->precision == PF_FP64 ? 8 : 16);
+        if (c->tile_j != 4 && c->tile_j != 8 && c->tile_j != 16)
+            return fail(PF_ERR_ARG, "tile_j must be 4, 8 or 16 (got %d)", c->tile_j);
+    }
     c->tile_x = d->tile_x > 0 ? d->tile_x : 0;
     c->esz = d->precision == PF_FP64 ? 8 : 4;
     c->sponge = d->sponge_x != nullptr;
@@ -311,6 +318,7 @@ template Mesh<double> pf::make_view<double>(const pf_ctx*, int);
 template <class R, int MODE>
 static int launch_sweep_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
     if (c->kernel == PF_KERNEL_TILED) return launch_tiled<R, MODE>(c, m, s);
+    if (c->kernel == PF_KERNEL_STRIP) return launch_strip<R, MODE>(c, m, s);
     dim3 blk(64, 4, 1);
     dim3 grd((unsigned)((c->nz + 63) / 64), (unsigned)((c->ny + 3) / 4), (unsigned)c->nx);
     if (c->sponge) hipLaunchKernelGGL((k_sweep_naive<R, MODE, true>), grd, blk, 0, s, m);

@@ -31,6 +31,16 @@ def test_tiled_equals_naive_on_golden_meshes(name, precision):
         assert np.array_equal(a[f], b[f]), f
 
 
+@pytest.mark.parametrize("name", ["default_json_100", "nonuniform_json_60"])
+@pytest.mark.parametrize("rows", [4, 8])
+def test_strip_equals_naive(name, rows):
+    spec = spec_from_golden(golden(name))
+    a = _fields(spec, "fp32", "naive")
+    b = _fields(spec, "fp32", "strip", tile_j=rows)
+    for f in a:
+        assert np.array_equal(a[f], b[f]), f
+
+
 def test_fp64_tiled_bitwise_vs_reference():
     for name in CASES:
         d = golden(name)
