@@ -27,23 +27,6 @@ struct StripMap {
     int nkt, nsg, nxc, xchunk;  // z tiles, strip groups (of W strips), x chunks, planes per chunk
 };
 
-// DPP whole-wavefront shifts (gfx9 family).  A lane with no source keeps
-// its own value; only halo lanes see that.
-PF_DEV int dpp_from_below(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
-PF_DEV int dpp_from_above(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false); }
-PF_DEV float lane_below(float v) { return __int_as_float(dpp_from_below(__float_as_int(v))); }
-PF_DEV float lane_above(float v) { return __int_as_float(dpp_from_above(__float_as_int(v))); }
-PF_DEV double lane_below(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = dpp_from_below((int)(b & 0xffffffffLL)), hi = dpp_from_below((int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-PF_DEV double lane_above(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = dpp_from_above((int)(b & 0xffffffffLL)), hi = dpp_from_above((int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
 template <class R, int MODE, bool SPONGE, int NR, int W>
 __global__ __launch_bounds__(64 * W, (MODE == MODE_FP32 ? 4 : 2)) void k_sweep_strip(Mesh<R> m, StripMap sm) {
     constexpr bool EXACT = MODE == MODE_FP64;

@@ -5,33 +5,43 @@
 // each side: lanes 0 and 63 and rows 0 and TJ-1 compute the stresses their
 // neighbours need but write nothing, so every thread runs the same code and
 // no halo special-casing is needed.  Per plane each thread
-//   1. loads its own column of u at plane i+1 (one coalesced load per
-//      component) and its material index into registers and LDS;
+//   1. takes its own column of u at plane i+1 (loaded two planes earlier)
+//      and its material index into registers and LDS;
 //   2. computes the normal stresses N = (T1, T2, T3) at plane i+1 and the
 //      shear stresses T4, T5, T6 at plane i from register values, LDS
-//      neighbours along y and cross-lane neighbours along z;
+//      neighbours along y and DPP lane shifts along z;
 //   3. publishes the stresses its y neighbours need through LDS;
 //   4. forms the displacement update of plane i (stage I) and writes it,
 //      plus the z = max Mur face cell above it when it is the top cell.
 // Own-column values of the previous plane (u, N, T5, T6) stay in registers,
 // so every u value is read from HBM once per step and written once.
 //
-// LDS hazards: su/sid of a plane parity are rewritten at the top of the
-// iteration two planes later, before that iteration's first barrier, so
-// they are only read before the current iteration's second barrier;
-// sn2/ss4/ss6 are rewritten after a first barrier and may be read after it.
+// Latency: the march is unrolled by two planes with one register set per
+// parity, so the loads of plane i+3 (current level) and of plane i+2
+// (previous level) are issued while plane i is computed and no register
+// holding a load in flight is ever copied.  Nothing inside the loop reads
+// global memory with a per-plane wait: the per-plane x spacings (and sponge
+// terms) are staged in LDS once per block.
+//
+// LDS hazards: su/sid of a parity are rewritten at the top of the plane two
+// planes later, before that plane's first barrier, so they are only read
+// between a plane's first and second barrier; sn2/ss4/ss6 are rewritten
+// after a first barrier and may be read after the second.
 //
 // The arithmetic is the shared per-value formulas of pf_formulas.hpp, fed
 // the same operands as the naive kernel, so both sweeps produce identical
 // bits.  A chunk starting at plane ib first runs one warm-up plane (ib-1)
 // that only builds the register/LDS state.
 #pragma once
+#include <type_traits>
+
 #include "pf_runtime.hpp"
 
 namespace pf {
 
 constexpr int TK = 64;        // lanes along z
 constexpr int KOUT = TK - 2;  // output columns per tile
+constexpr int XT = 256;       // x-table entries: planes per chunk + 3
 
 struct TileMap {
     int nkt, njt, nxc, xchunk;
@@ -46,22 +56,30 @@ PF_DEV int xcd_remap(int b, int nb) {
     return x * q + (x < rem ? x : rem) + l;
 }
 
-template <class R> PF_DEV R from_lane_below(R v) { return __shfl_up(v, 1, 64); }   // value of lane-1 (z-1)
-template <class R> PF_DEV R from_lane_above(R v) { return __shfl_down(v, 1, 64); } // value of lane+1 (z+1)
-
-// Own column of one plane: u components and material index.
-template <class R>
-struct Col {
-    R x, y, z;
-    int id;
-};
+// DPP whole-wavefront shifts (gfx9 family: wave_shr:1 / wave_shl:1).  A lane
+// with no source keeps its own value, as __shfl_up/__shfl_down do; only
+// halo lanes see that.
+PF_DEV int dpp_from_below(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
+PF_DEV int dpp_from_above(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false); }
+PF_DEV float lane_below(float v) { return __int_as_float(dpp_from_below(__float_as_int(v))); }
+PF_DEV float lane_above(float v) { return __int_as_float(dpp_from_above(__float_as_int(v))); }
+PF_DEV double lane_below(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_from_below((int)(b & 0xffffffffLL)), hi = dpp_from_below((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+PF_DEV double lane_above(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_from_above((int)(b & 0xffffffffLL)), hi = dpp_from_above((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
 // Plane accesses go through buffer resources: one SGPR descriptor per array
 // (built once per block), the plane selected by the scalar byte offset
 // ip * plane * sizeof(R) and the cell by a per-lane 32-bit byte offset that
 // does not change along the march, so a load or store costs no per-lane
 // address arithmetic.  Arrays must be smaller than 4 GiB (the launcher
-// checks); accesses beyond an array's range read 0 / are dropped.
+// checks); every access made here lies inside its array.
 typedef __amdgpu_buffer_rsrc_t Rsrc;
 // descriptor word 3 for untyped (raw) buffer access on gfx9-family GPUs
 constexpr int PF_RSRC_FLAGS = 0x00020000;
@@ -87,6 +105,9 @@ PF_DEV void bst(Rsrc r, unsigned off, unsigned soff, double v) {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, (int)off, (int)soff, 0);
 }
 
+// A block-uniform value kept in a scalar register (plane offsets).
+PF_DEV unsigned uni(unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlane((int)v); }
+
 // Descriptors of every array the sweep streams.
 struct Bufs {
     Rsrc c0, c1, c2, n0, n1, n2, cl0, cl1, cl2, nl0, nl1, nl2, ids;
@@ -102,37 +123,98 @@ template <class R> PF_DEV Bufs make_bufs(const Mesh<R>& m) {
     return b;
 }
 
+// Own column of one plane: u components and material index.
 template <class R>
-PF_DEV Col<R> load_col(const Mesh<R>& m, const Bufs& bf, int ip, bool valid, unsigned col, int k, R w) {
-    const bool pin = ip >= 0 && ip < m.nx;
-    const bool in = valid && pin;
-    const unsigned pc = pin ? (unsigned)ip : 0u;
-    const unsigned boff = col * (unsigned)sizeof(R);
-    const unsigned soff = pc * (unsigned)m.plane * (unsigned)sizeof(R);
-    const R x = bld(bf.c0, boff, soff, R(0));
-    const R y = bld(bf.c1, boff, soff, R(0));
-    const R z = bld(bf.c2, boff, soff, R(0));
-    const int id = bld_u8(bf.ids, col, pc * (unsigned)m.plane);
+struct Col {
+    R x, y, z;
+    int id;
+};
+
+// The same, as loaded: not yet masked for out-of-mesh lanes nor given the
+// source value.  Kept separate so that no select waits on a load in flight.
+template <class R>
+struct Raw {
+    R x, y, z;
+    uint8_t id;
+};
+
+// Current-level column of plane p, which the caller keeps inside the mesh.
+// Loads are never conditional, so that no register with a load in flight is
+// merged with another value (which would force a wait and a copy).
+template <class R>
+PF_DEV Raw<R> load_raw(const Bufs& bf, int p, unsigned plane, unsigned boff, unsigned col) {
+    Raw<R> r;
+    const unsigned soff = uni((unsigned)p * plane * (unsigned)sizeof(R));
+    r.x = bld(bf.c0, boff, soff, R(0));
+    r.y = bld(bf.c1, boff, soff, R(0));
+    r.z = bld(bf.c2, boff, soff, R(0));
+    r.id = __builtin_amdgcn_raw_buffer_load_b8(bf.ids, (int)col, (int)uni((unsigned)p * plane), 0);
+    return r;
+}
+
+// Out-of-mesh lanes and planes read as 0; the wall line uz(0, j, 0) reads
+// the source.
+template <class R>
+PF_DEV Col<R> settle(const Raw<R>& r, bool valid, bool src, R w) {
     Col<R> c;
-    c.x = in ? x : R(0);
-    c.y = in ? y : R(0);
-    c.z = in ? ((ip == 0 && k == 0) ? w : z) : R(0);
-    c.id = in ? id : 0;
+    c.x = valid ? r.x : R(0);
+    c.y = valid ? r.y : R(0);
+    c.z = valid ? (src ? w : r.z) : R(0);
+    c.id = valid ? (int)r.id : 0;
     return c;
 }
 
+// Values the displacement update of a plane reads at the node itself: the
+// previous level (hi, and lo in FP32C) and the current level's lo part.
+template <class R>
+struct Old {
+    R o0, o1, o2, ol0, ol1, ol2, cl0, cl1, cl2;
+};
+
+template <class R, bool COMP>
+PF_DEV Old<R> load_old(const Bufs& bf, int i, unsigned plane, unsigned boff) {
+    const unsigned soff = uni((unsigned)i * plane * (unsigned)sizeof(R));
+    Old<R> o;
+    o.o0 = bld(bf.n0, boff, soff, R(0));
+    o.o1 = bld(bf.n1, boff, soff, R(0));
+    o.o2 = bld(bf.n2, boff, soff, R(0));
+    if constexpr (COMP) {
+        o.ol0 = bld(bf.nl0, boff, soff, R(0));
+        o.ol1 = bld(bf.nl1, boff, soff, R(0));
+        o.ol2 = bld(bf.nl2, boff, soff, R(0));
+        o.cl0 = bld(bf.cl0, boff, soff, R(0));
+        o.cl1 = bld(bf.cl1, boff, soff, R(0));
+        o.cl2 = bld(bf.cl2, boff, soff, R(0));
+    } else {
+        o.ol0 = o.ol1 = o.ol2 = o.cl0 = o.cl1 = o.cl2 = R(0);
+    }
+    return o;
+}
+
+// Per-plane x data staged in LDS: hs = staggered spacing (sx[p-1], 1 off
+// its range), hf = full spacing (fx[p], 1 off its range), dp = sponge term.
+template <class R>
+struct XPlane {
+    R hs, hf, dp, pad;
+};
+
 template <class R> PF_DEV R sel(bool c, R a, R b) { return c ? a : b; }
 
-// Own-cell store through the array's buffer descriptor, then the z = max
-// face cell above when this is the top cell (see put_with_zface).
+// Own-cell store through the array's buffer descriptor, then, for the top
+// cell of the column (k = Lz - 1), the z = max Mur face cell above it:
+// face = mur(cur, s0, cur above), the cell above being the next lane.
 template <int Q, int MODE, class R>
-PF_DEV void put_tile(const Mesh<R>& m, const Bufs& bf, unsigned boff, unsigned soff, int i, int j, int k,
-                     V<R> s0, V<R> cur, R w) {
-    bst(Q == 0 ? bf.n0 : (Q == 1 ? bf.n1 : bf.n2), boff, soff, s0.h);
-    if constexpr (MODE == MODE_FP32C) bst(Q == 0 ? bf.nl0 : (Q == 1 ? bf.nl1 : bf.nl2), boff, soff, s0.l);
-    const int Lz = Ext<Q>::Lz(m.nz);
-    if (k == Lz - 1)
-        put<Q, MODE>(m, i, j, Lz, mur_map<MODE>(cur, m.mur[9 + Q], m.mur_lo[9 + Q], s0, cur_q<Q>(m, i, j, Lz, w)));
+PF_DEV void put_tile(const Mesh<R>& m, const Bufs& bf, unsigned boff, unsigned soff, bool top, V<R> s0, V<R> cur,
+                     V<R> above) {
+    const Rsrc nh = Q == 0 ? bf.n0 : (Q == 1 ? bf.n1 : bf.n2);
+    const Rsrc nl = Q == 0 ? bf.nl0 : (Q == 1 ? bf.nl1 : bf.nl2);
+    bst(nh, boff, soff, s0.h);
+    if constexpr (MODE == MODE_FP32C) bst(nl, boff, soff, s0.l);
+    if (top) {
+        const V<R> f = mur_map<MODE>(cur, m.mur[9 + Q], m.mur_lo[9 + Q], s0, above);
+        bst(nh, boff + (unsigned)sizeof(R), soff, f.h);
+        if constexpr (MODE == MODE_FP32C) bst(nl, boff + (unsigned)sizeof(R), soff, f.l);
+    }
 }
 
 template <class R, int MODE, bool SPONGE, int TJ>
@@ -145,6 +227,7 @@ __global__ __launch_bounds__(TK * TJ, (MODE == MODE_FP32 ? 4 : 2)) void k_sweep_
     __shared__ R ss6[TJ][TK];        // T6 of the current plane
     __shared__ uint8_t sid[2][TJ][TK];
     __shared__ R smat[PF_TILED_MAX_MATERIALS * MAT_STRIDE];
+    __shared__ XPlane<R> sxp[XT];    // planes ib-1 .. ie+1
 
     const int lane = threadIdx.x, row = threadIdx.y;
     const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
@@ -165,9 +248,18 @@ __global__ __launch_bounds__(TK * TJ, (MODE == MODE_FP32 ? 4 : 2)) void k_sweep_
     const R w = (R)m.wave[tt];
     const int rp = row + 1 < TJ ? row + 1 : TJ - 1;
     const int rm = row > 0 ? row - 1 : 0;
-    const int lp = lane + 1 < TK ? lane + 1 : TK - 1;
+    const int tid = row * TK + lane;
 
-    for (int e = row * TK + lane; e < m.nmat * MAT_STRIDE; e += TK * TJ) smat[e] = m.mat[e];
+    for (int e = tid; e < m.nmat * MAT_STRIDE; e += TK * TJ) smat[e] = m.mat[e];
+    for (int e = tid; e < ie - ib + 3; e += TK * TJ) {
+        const int p = ib - 1 + e;
+        XPlane<R> x;
+        x.hs = (p >= 1 && p <= m.nx - 2) ? m.sx[p - 1] : R(1);
+        x.hf = (p >= 0 && p <= m.nx - 2) ? m.fx[p] : R(1);
+        x.dp = (SPONGE && p >= 0 && p < m.nx) ? m.dpx[p] : R(0);
+        x.pad = R(0);
+        sxp[e] = x;
+    }
 
     // Column constants.  Every k = 0 (free surface) variant of the
     // reference's expressions is the interior expression with other
@@ -190,71 +282,82 @@ __global__ __launch_bounds__(TK * TJ, (MODE == MODE_FP32 ? 4 : 2)) void k_sweep_
     const R s6_h1 = sel(ks, hfy, m.fx0);
     const R uy_g2 = sel(ks, hfy, m.fy0);
     const R uz_g3 = sel(ks, hfz, R(1));
+    R dpyz = R(0), dpz_k = R(0);
+    if constexpr (SPONGE) {
+        dpyz = m.dpy[j < 0 ? 0 : (j < m.ny ? j : m.ny - 1)];
+        dpz_k = m.dpz[k < 0 ? 0 : (k < m.nz ? k : m.nz - 1)];
+    }
     const unsigned col = valid ? (unsigned)(j * m.pz + k) : 0u;
     const unsigned boff = col * (unsigned)sizeof(R);
-    // non-face cells per component (i-independent part)
+    const unsigned plane = (unsigned)m.plane;
+    // non-face cells per component (i-independent part); top = z-face below
     const bool wx = writer && jn && k <= m.nz - 2;
     const bool wy = writer && j >= 1 && j <= m.ny - 3 && k <= m.nz - 2;
     const bool wz = writer && jn && k <= m.nz - 3;
+    const bool top_xy = k == m.nz - 2;
+    const bool top_z = k == m.nz - 3;
 
     const Bufs bf = make_bufs(m);
-    Col<R> c0 = load_col(m, bf, ib - 1, valid, col, k, w);
-    Col<R> nxt = load_col(m, bf, ib, valid, col, k, w);  // prefetched plane i+1
-    {
-        const int bb = (ib - 1) & 1;
-        su[bb][0][row][lane] = c0.x;
-        su[bb][1][row][lane] = c0.y;
-        su[bb][2][row][lane] = c0.z;
-        sid[bb][row][lane] = (uint8_t)c0.id;
-    }
+    // prologue: plane ib-1 (warm-up, previous-plane buffer 1), then the
+    // loads in flight for the first plane of each parity
+    // Loads of planes the chunk does not need are clamped to plane ie (or
+    // nx-1), which the block reads anyway; planes outside the mesh are then
+    // zeroed by settle().
+    const int pmax = ie < m.nx ? ie : m.nx - 1;
+    Col<R> c0 = settle(load_raw<R>(bf, ib >= 1 ? ib - 1 : 0, plane, boff, col), valid && ib >= 1,
+                       ib - 1 == 0 && k == 0, w);
+    // issue order ra, rb, ob (kept by the scheduling barriers) matches the
+    // steady state, so the loop needs no wait for more than it uses
+    Raw<R> ra = load_raw<R>(bf, ib, plane, boff, col);
+    __builtin_amdgcn_sched_barrier(0);
+    Raw<R> rb = load_raw<R>(bf, ib + 1 <= pmax ? ib + 1 : pmax, plane, boff, col);
+    __builtin_amdgcn_sched_barrier(0);
+    Old<R> ob = load_old<R, COMP>(bf, ib, plane, boff);
+    __builtin_amdgcn_sched_barrier(0);
+    Old<R> oa = {};  // the warm-up plane loads plane ib+1 into it
+    su[1][0][row][lane] = c0.x;
+    su[1][1][row][lane] = c0.y;
+    su[1][2][row][lane] = c0.z;
+    sid[1][row][lane] = (uint8_t)c0.id;
+
     R n1_0 = 0, n2_0 = 0, n3_0 = 0;  // N at plane i
     R s5_m = 0, s6_m = 0;            // T5, T6 at plane i-1
 
-    for (int i = ib - 1; i < ie; ++i) {
+    // One plane i: H is the LDS parity of plane i+1, rin holds the raw
+    // column of plane i+1 and receives plane i+3, oin holds the node values
+    // of plane i and receives plane i+2.
+    auto plane_step = [&](int i, auto Hc, Raw<R>& rin, Old<R>& oin) {
+        constexpr int H = decltype(Hc)::value;
         const int ip = i + 1;
-        const int b = ip & 1;
-        const Col<R> c1 = nxt;
-        nxt = load_col(m, bf, ip + 1, valid, col, k, w);  // in flight across this plane
-        // previous-level values the update of plane i needs, also in flight
-        const unsigned soff = (unsigned)(i < 0 ? 0 : i) * (unsigned)m.plane * (unsigned)sizeof(R);
-        const R o0 = bld(bf.n0, boff, soff, R(0));
-        const R o1 = bld(bf.n1, boff, soff, R(0));
-        const R o2 = bld(bf.n2, boff, soff, R(0));
-        R ol0 = 0, ol1 = 0, ol2 = 0, cl0 = 0, cl1 = 0, cl2 = 0;
-        if constexpr (COMP) {
-            ol0 = bld(bf.nl0, boff, soff, R(0));
-            ol1 = bld(bf.nl1, boff, soff, R(0));
-            ol2 = bld(bf.nl2, boff, soff, R(0));
-            cl0 = bld(bf.cl0, boff, soff, R(0));
-            cl1 = bld(bf.cl1, boff, soff, R(0));
-            const R c2 = bld(bf.cl2, boff, soff, R(0));
-            cl2 = (i == 0 && k == 0) ? R(0) : c2;
-        }
-        su[b][0][row][lane] = c1.x;
-        su[b][1][row][lane] = c1.y;
-        su[b][2][row][lane] = c1.z;
-        sid[b][row][lane] = (uint8_t)c1.id;
+        const Col<R> c1 = settle(rin, valid && ip < m.nx, ip == 0 && k == 0, w);
+        su[H][0][row][lane] = c1.x;
+        su[H][1][row][lane] = c1.y;
+        su[H][2][row][lane] = c1.z;
+        sid[H][row][lane] = (uint8_t)c1.id;
         __syncthreads();
+        // issued after the barrier, so the registers of rin are free again
+        rin = load_raw<R>(bf, ip + 2 <= pmax ? ip + 2 : pmax, plane, boff, col);
 
-        const R uy1_jm = su[b][1][rm][lane];
-        const R uz1_km = from_lane_below(c1.z);
-        const R uy0_kp = from_lane_above(c0.y);
-        const R ux0_kp = from_lane_above(c0.x);
-        const R uz0_jp = su[1 - b][2][rp][lane];
-        const R ux0_jp = su[1 - b][0][rp][lane];
-        const int id_jp = sid[1 - b][rp][lane];     // (i, j+1, k)
-        const int id_kp = sid[1 - b][row][lp];      // (i, j, k+1)
-        const int id_jkp = sid[1 - b][rp][lp];      // (i, j+1, k+1)
-        const int id1_kp = sid[b][row][lp];         // (i+1, j, k+1)
-        const int id1_jp = sid[b][rp][lane];        // (i+1, j+1, k)
+        const XPlane<R> xq0 = sxp[i - ib + 1];  // x data of plane i
+        const XPlane<R> xq1 = sxp[ip - ib + 1];
+        const R uy1_jm = su[H][1][rm][lane];
+        const R uz1_km = lane_below(c1.z);
+        const R uy0_kp = lane_above(c0.y);
+        const R ux0_kp = lane_above(c0.x);
+        const R uz0_jp = su[1 - H][2][rp][lane];
+        const R ux0_jp = su[1 - H][0][rp][lane];
+        const int id_jp = sid[1 - H][rp][lane];     // (i, j+1, k)
+        const int id1_jp = sid[H][rp][lane];        // (i+1, j+1, k)
+        const int id_kp = dpp_from_above(c0.id);    // (i, j, k+1)
+        const int id_jkp = dpp_from_above(id_jp);   // (i, j+1, k+1)
+        const int id1_kp = dpp_from_above(c1.id);   // (i+1, j, k+1)
 
         // normal stresses at plane ip (base_solver.py:329-351, 408-418)
         R n1_1, n2_1, n3_1;
         {
             const R* c = smat + MAT_STRIDE * c1.id;
-            const R hx = sel(ks, ip >= 1 ? m.sx[ip - 1] : R(1), m.sx0);
-            normal3<EXACT>(c, c1.x - c0.x, c1.y - uy1_jm, c1.z - sel(ks, uz1_km, R(0)), hx, hy_n, hz_n,
-                           n1_1, n2_1, n3_1);
+            normal3<EXACT>(c, c1.x - c0.x, c1.y - uy1_jm, c1.z - sel(ks, uz1_km, R(0)), sel(ks, xq1.hs, m.sx0), hy_n,
+                           hz_n, n1_1, n2_1, n3_1);
             const bool nv = ip >= 1 && ip <= m.nx - 2 && jn && kf;
             n1_1 = sel(nv, n1_1, R(0));
             n2_1 = sel(nv, n2_1, R(0));
@@ -262,7 +365,7 @@ __global__ __launch_bounds__(TK * TJ, (MODE == MODE_FP32 ? 4 : 2)) void k_sweep_
         }
         // shear stresses at plane i (base_solver.py:353-372, 420-433)
         const bool ii = i >= 0 && i <= m.nx - 2;
-        const R fxi = sel(ks, ii ? m.fx[i] : R(1), m.fz0);
+        const R fxi = sel(ks, xq0.hf, m.fz0);
         R s4 = shear<EXACT>(smat[MAT_STRIDE * (ks ? id_jkp : id_jp) + M_C33], uy0_kp - c0.y, s4_h1,
                             uz0_jp - c0.z, s4_h2);
         R s5 = shear<EXACT>(smat[MAT_STRIDE * (ks ? id1_kp : c1.id) + M_C44], ux0_kp - c0.x, s5_h1,
@@ -271,54 +374,68 @@ __global__ __launch_bounds__(TK * TJ, (MODE == MODE_FP32 ? 4 : 2)) void k_sweep_
         s4 = sel(ii && i >= 1 && jf && kf, s4, R(0));
         s5 = sel(ii && jn && kf, s5, R(0));
         s6 = sel(ii && jf && kf, s6, R(0));
-        sn2[b][row][lane] = n2_1;
+        sn2[H][row][lane] = n2_1;
         ss4[row][lane] = s4;
         ss6[row][lane] = s6;
         __syncthreads();
 
-        const R n3_0_kp = from_lane_above(n3_0);
-        const R s4_km = from_lane_below(s4);
-        const R s5_km = from_lane_below(s5);
-        if (i >= ib) {
-            const R n2_0_jp = sn2[1 - b][rp][lane];
+        const R n3_0_kp = lane_above(n3_0);
+        const R s4_km = lane_below(s4);
+        const R s5_km = lane_below(s5);
+        if (i >= ib && i < ie) {
+            const R n2_0_jp = sn2[1 - H][rp][lane];
             const R s4_jm = ss4[rm][lane];
             const R s6_jm = ss6[rm][lane];
-            const R s = SPONGE ? damp_at(m, i, j < 0 ? 0 : (j < m.ny ? j : m.ny - 1),
-                                         k < 0 ? 0 : (k < m.nz ? k : m.nz - 1))
-                               : R(0);
-            const R gx = sel(ks, i >= 1 && i <= m.nx - 2 ? m.sx[i - 1] : R(1), m.sx0);
+            const R s = SPONGE ? (xq0.dp + dpyz) + dpz_k : R(0);
+            const R gx = sel(ks, xq0.hs, m.sx0);
+            const unsigned soff = uni((unsigned)i * plane * (unsigned)sizeof(R));
+            const R cl2 = (i == 0 && k == 0) ? R(0) : oin.cl2;
             // ux (base_solver.py:441-447, 495-501)
             {
-                const R dv = div3<EXACT>(n1_1 - n1_0, sel(ks, i <= m.nx - 2 ? m.fx[i] : R(1), m.fx0),
-                                         s6 - s6_jm, hy_n, s5 - sel(ks, s5_km, R(0)), hz_n);
-                const V<R> cur = {c0.x, cl0};
-                const V<R> s0 = leap_from<MODE, SPONGE>(cur, o0, ol0, smat[MAT_STRIDE * c1.id + M_COEF], dv, s);
-                if (wx && i <= m.nx - 3) put_tile<0, MODE>(m, bf, boff, soff, i, j, k, s0, cur, w);
+                const R dv = div3<EXACT>(n1_1 - n1_0, sel(ks, xq0.hf, m.fx0), s6 - s6_jm, hy_n,
+                                         s5 - sel(ks, s5_km, R(0)), hz_n);
+                const V<R> cur = {c0.x, oin.cl0};
+                const V<R> s0 =
+                    leap_from<MODE, SPONGE>(cur, oin.o0, oin.ol0, smat[MAT_STRIDE * c1.id + M_COEF], dv, s);
+                const V<R> above = {ux0_kp, COMP ? lane_above(oin.cl0) : R(0)};
+                if (wx && i <= m.nx - 3) put_tile<0, MODE>(m, bf, boff, soff, top_xy, s0, cur, above);
             }
             // uy (base_solver.py:449-455, 503-509)
             {
                 const R dv = div3<EXACT>(s6 - s6_m, gx, n2_0_jp - n2_0, uy_g2, s4 - sel(ks, s4_km, R(0)), hz_n);
-                const V<R> cur = {c0.y, cl1};
-                V<R> s0 = leap_from<MODE, SPONGE>(cur, o1, ol1, smat[MAT_STRIDE * id_jp + M_COEF], dv, s);
+                const V<R> cur = {c0.y, oin.cl1};
+                V<R> s0 = leap_from<MODE, SPONGE>(cur, oin.o1, oin.ol1, smat[MAT_STRIDE * id_jp + M_COEF], dv, s);
                 if (i == 0) s0 = {R(0), R(0)};
-                if (wy && i <= m.nx - 2) put_tile<1, MODE>(m, bf, boff, soff, i, j, k, s0, cur, w);
+                const V<R> above = {uy0_kp, COMP ? lane_above(oin.cl1) : R(0)};
+                if (wy && i <= m.nx - 2) put_tile<1, MODE>(m, bf, boff, soff, top_xy, s0, cur, above);
             }
             // uz (base_solver.py:457-463, 511-517)
             {
                 const R dv = div3<EXACT>(s5 - s5_m, gx, s4 - s4_jm, hy_n, n3_0_kp - n3_0, uz_g3);
                 const V<R> cur = {c0.z, cl2};
-                V<R> s0 = leap_from<MODE, SPONGE>(cur, o2, ol2, smat[MAT_STRIDE * (ks ? id_kp : c0.id) + M_COEF],
-                                                  dv, s);
+                V<R> s0 = leap_from<MODE, SPONGE>(cur, oin.o2, oin.ol2,
+                                                  smat[MAT_STRIDE * (ks ? id_kp : c0.id) + M_COEF], dv, s);
                 if (i == 0) s0 = {R(0), R(0)};
-                if (wz && i <= m.nx - 2) put_tile<2, MODE>(m, bf, boff, soff, i, j, k, s0, cur, w);
+                const V<R> above = {lane_above(c0.z), COMP ? lane_above(cl2) : R(0)};
+                if (wz && i <= m.nx - 2) put_tile<2, MODE>(m, bf, boff, soff, top_z, s0, cur, above);
             }
         }
+        oin = load_old<R, COMP>(bf, i + 2 < ie ? i + 2 : ie - 1, plane, boff);
         c0 = c1;
         n1_0 = n1_1;
         n2_0 = n2_1;
         n3_0 = n3_1;
         s5_m = s5;
         s6_m = s6;
+    };
+
+    // planes ib-1, ib+1, ... use parity 0 for plane i+1; ib, ib+2, ... parity 1
+    // (an odd count of planes ends with a parity-1 pass over plane ie that
+    // only loads and computes: every path through the loop body issues the
+    // same loads, so no wait is made for the shortest of several paths)
+    for (int i = ib - 1; i < ie; i += 2) {
+        plane_step(i, std::integral_constant<int, 0>(), ra, oa);
+        plane_step(i + 1, std::integral_constant<int, 1>(), rb, ob);
     }
 }
 
@@ -333,6 +450,7 @@ inline TileMap tile_map(const pf_ctx* c, int tj, int xchunk) {
         if (want < 1) want = 1;
         xchunk = (int)((c->nx + want - 1) / want);
         if (xchunk < 16) xchunk = 16;
+        if (xchunk > XT - 3) xchunk = XT - 3;
     }
     if (xchunk > c->nx) xchunk = c->nx;
     tm.xchunk = xchunk;
@@ -345,6 +463,7 @@ int launch_tiled_s(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
     const int tj = c->tile_j;
     if ((unsigned long long)c->cells * sizeof(R) >= (1ULL << 32))
         return fail(PF_ERR_ARG, "tiled sweep: field arrays must be < 4 GiB (use the naive kernel)");
+    if (c->tile_x > XT - 3) return fail(PF_ERR_ARG, "tile_x must be <= %d (got %d)", XT - 3, c->tile_x);
     TileMap tm = tile_map(c, tj, c->tile_x);
     const long long nb = (long long)tm.nkt * tm.njt * tm.nxc;
     if (nb > 0x7fffffffLL) return fail(PF_ERR_ARG, "too many tiles");

@@ -49,7 +49,7 @@ def test_fp64_tiled_bitwise_vs_reference():
             assert np.array_equal(out[f], d[f]), (name, f)
 
 
-@pytest.mark.parametrize("tile_j,tile_x", [(4, 0), (8, 7), (16, 0), (16, 33)])
+@pytest.mark.parametrize("tile_j,tile_x", [(4, 0), (4, 2), (8, 1), (8, 7), (16, 0), (16, 33), (16, 70)])
 @pytest.mark.parametrize("precision", ["fp32", "fp32c"])
 def test_tiled_shapes_on_crystal(tile_j, tile_x, precision):
     # z extent not a multiple of the 62 output lanes, y not of the row count
