@@ -92,7 +92,7 @@ def bench_plain(args):
     spec, g, m = crystal_spec((512, 512, 256), 24, 8, 3, steps)
     nodes = spec.cells
     ctx = fdtd.Context(spec, precision=args.precision, kernel=args.kernel, tile_j=args.tile_j,
-                       tile_x=args.tile_x)
+                       tile_x=args.tile_x, tile_z=args.tile_z)
     ctx.step(args.warmup)
     ctx.sync()
     t0 = time.perf_counter()
@@ -118,6 +118,7 @@ def bench_plain(args):
         "config": {"workload": "plain %s sweep, 512x512x256-cell phononic crystal (513x513x257 nodes), "
                                "lattice a=24 r=8 cells, seed 3, gauss 2 MHz source" % args.precision,
                    "nodes": nodes, "kernel": args.kernel, "tile_j": args.tile_j, "tile_x": args.tile_x,
+                   "tile_z": args.tile_z,
                    "fill_fraction": float(m.ids.mean())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
@@ -136,7 +137,8 @@ def main():
     ap.add_argument("--workload", default="plain", choices=["plain"])
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32c", "fp64"])
     ap.add_argument("--kernel", default="auto", choices=["auto", "naive", "tiled", "strip"])
-    ap.add_argument("--tile-j", type=int, default=0, help="tiled sweep rows per block (4, 8, 16)")
+    ap.add_argument("--tile-j", type=int, default=0, help="tiled sweep rows per block (4..32)")
+    ap.add_argument("--tile-z", type=int, default=0, help="tiled sweep lanes along z per row (32, 64)")
     ap.add_argument("--tile-x", type=int, default=0, help="tiled sweep x planes per block")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()

@@ -84,9 +84,14 @@ typedef struct pf_desc {
                                   (ni == 0: no probes)                          */
     int32_t record_every;      /* probe frame after every k-th step            */
     int32_t probe_capacity;    /* ring size in frames                          */
-    int32_t tile_j;            /* rows per block (tiled: 4/8/16) or per
+    int32_t tile_j;            /* rows per block (tiled: 4/8/16 with tile_z
+                                  64, 8/16/32 with tile_z 32) or per
                                   wavefront (strip: 4/8); 0 = default          */
     int32_t tile_x;            /* x planes per block (0 = automatic)           */
+    int32_t tile_z;            /* tiled: lanes along z per tile row, 64 (one
+                                  row per wavefront) or 32 (two rows per
+                                  wavefront); 0 = the one with fewer idle
+                                  lanes for this mesh                          */
 } pf_desc;
 
 /* Library / device identification. */

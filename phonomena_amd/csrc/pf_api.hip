@@ -153,26 +153,45 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
     c->nx = d->nx;
     c->ny = d->ny;
     c->nz = d->nz;
-    c->pz = (d->nz + 31) / 32 * 32;
+    // rows are not padded: with z contiguous the sweep streams whole planes,
+    // and padding rows to a cache-line multiple would only add traffic
+    c->pz = d->nz;
     c->plane = (long long)d->ny * c->pz;
     c->cells = (long long)d->nx * c->plane;
     c->precision = d->precision;
     const bool few_mats = d->n_materials <= PF_TILED_MAX_MATERIALS;
-    const bool small = (unsigned long long)c->cells * (d->precision == PF_FP64 ? 8 : 4) < (1ULL << 32);
+    // the tiled sweep addresses a whole level with 32-bit offsets
+    const bool small = (unsigned long long)c->cells * (d->precision == PF_FP64 ? 8 : 4) *
+                           (d->precision == PF_FP32C ? 6 : 3) < (1ULL << 32);
     c->kernel = d->kernel == PF_KERNEL_AUTO ? (few_mats && small ? PF_KERNEL_TILED : PF_KERNEL_NAIVE) : d->kernel;
     if ((c->kernel == PF_KERNEL_TILED || c->kernel == PF_KERNEL_STRIP) && !few_mats)
         return fail(PF_ERR_ARG, "the tiled and strip sweeps hold at most %d materials (got %d)",
                     PF_TILED_MAX_MATERIALS, d->n_materials);
+    c->tile_x = d->tile_x > 0 ? d->tile_x : 0;
     if (c->kernel == PF_KERNEL_STRIP) {
         c->tile_j = d->tile_j > 0 ? d->tile_j : 4;
         if (c->tile_j != 4 && c->tile_j != 8)
             return fail(PF_ERR_ARG, "strip rows must be 4 or 8 (got %d)", c->tile_j);
     } else {
-        c->tile_j = d->tile_j > 0 ? d->tile_j : (d->precision == PF_FP64 ? 8 : 16);
-        if (c->tile_j != 4 && c->tile_j != 8 && c->tile_j != 16)
-            return fail(PF_ERR_ARG, "tile_j must be 4, 8 or 16 (got %d)", c->tile_j);
+        // default rows: 1024 threads per block (512 in FP64, whose registers
+        // allow no more)
+        const bool f64 = d->precision == PF_FP64;
+        const int tj64 = f64 ? 8 : 16, tj32 = f64 ? 16 : 32;
+        int kz = d->tile_z;
+        if (kz != 0 && kz != 32 && kz != 64) return fail(PF_ERR_ARG, "tile_z must be 0, 32 or 64 (got %d)", kz);
+        if (kz == 0) {
+            if (d->tile_j > 0)
+                kz = d->tile_j == 32 ? 32 : 64;
+            else
+                kz = pf::tiled_slots(c, tj32, 32) < pf::tiled_slots(c, tj64, 64) ? 32 : 64;
+        }
+        c->tile_z = kz;
+        c->tile_j = d->tile_j > 0 ? d->tile_j : (kz == 64 ? tj64 : tj32);
+        const bool ok = kz == 64 ? (c->tile_j == 4 || c->tile_j == 8 || c->tile_j == 16)
+                                 : (c->tile_j == 8 || c->tile_j == 16 || c->tile_j == 32);
+        if (!ok) return fail(PF_ERR_ARG, "tile_j must be 4, 8 or 16 with tile_z 64, 8, 16 or 32 with tile_z 32 (got %d, %d)",
+                             c->tile_j, kz);
     }
-    c->tile_x = d->tile_x > 0 ? d->tile_x : 0;
     c->esz = d->precision == PF_FP64 ? 8 : 4;
     c->sponge = d->sponge_x != nullptr;
     c->dt = d->dt;
@@ -180,17 +199,19 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
     HIPCHK(hipGetDevice(&c->dev));
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
 
-    // fields: two levels x three components (+ lo parts in compensated mode)
+    // fields: two levels x three components (+ lo parts in compensated mode),
+    // each level one allocation
     const size_t bytes = (size_t)c->cells * c->esz;
-    for (int l = 0; l < 2; l++)
+    const int per = c->precision == PF_FP32C ? 6 : 3;
+    for (int l = 0; l < 2; l++) {
+        HIPCHK(hipMalloc(&c->lvlbase[l], bytes * per));
+        HIPCHK(hipMemset(c->lvlbase[l], 0, bytes * per));
+        char* b = (char*)c->lvlbase[l];
         for (int q = 0; q < 3; q++) {
-            HIPCHK(hipMalloc(&c->lvl[l][q], bytes));
-            HIPCHK(hipMemset(c->lvl[l][q], 0, bytes));
-            if (c->precision == PF_FP32C) {
-                HIPCHK(hipMalloc(&c->lvlo[l][q], bytes));
-                HIPCHK(hipMemset(c->lvlo[l][q], 0, bytes));
-            }
+            c->lvl[l][q] = b + q * bytes;
+            if (c->precision == PF_FP32C) c->lvlo[l][q] = b + (3 + q) * bytes;
         }
+    }
     // material index map, padded like the fields
     HIPCHK(hipMalloc((void**)&c->ids, (size_t)c->cells));
     HIPCHK(hipMemset(c->ids, 0, (size_t)c->cells));
@@ -257,7 +278,7 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
 void pf::ctx_free(pf_ctx* c) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
     for (int l = 0; l < 2; l++)
-        for (int q = 0; q < 3; q++) { F(c->lvl[l][q]); F(c->lvlo[l][q]); }
+        F(c->lvlbase[l]);
     F(c->ids);
     F(c->mat); F(c->fx); F(c->fy); F(c->fz); F(c->sx); F(c->sy); F(c->sz);
     F(c->dpx); F(c->dpy); F(c->dpz);

@@ -2,11 +2,12 @@
 // naive sweep, the boundary-face kernel, probes, energy audit, pack/unpack.
 //
 // Layout in HBM (one mesh): each displacement component is a dense
-// (nx, ny, pz) array, z contiguous, pz = nz rounded up to 32 elements, for
-// two time levels (ping-pong).  Cells outside a component's reference extent
-// (ux has nx-1 planes, uy ny-1 rows, uz nz-1 columns) are allocated, never
-// written and stay zero.  Material: one uint8 index per node in the same
-// padded layout, and a 16-value row per material.
+// (nx, ny, pz) array, z contiguous, row stride pz (= nz: unpadded), for two
+// time levels (ping-pong); a level's arrays lie back to back in one
+// allocation.  Cells outside a component's reference extent (ux has nx-1
+// planes, uy ny-1 rows, uz nz-1 columns) are allocated, never written and
+// stay zero.  Material: one uint8 index per node in the same layout, and a
+// 16-value row per material.
 //
 // Semantics of one step (what the reference does in base_solver.py:245-256):
 //   1. the wall line uz[0, :, 0] takes the source value w(tt); here the value

@@ -9,17 +9,20 @@
 #define PF_TILED_MAX_MATERIALS 16
 
 // One mesh on one device.  Two levels of each displacement component are
-// kept (ping-pong); `cur` indexes the current level.
+// kept (ping-pong); `cur` indexes the current level.  Each level is one
+// allocation holding its arrays back to back (ux, uy, uz, then the lo parts
+// in FP32C), so one buffer descriptor covers a whole level.
 struct pf_ctx {
     int nx = 0, ny = 0, nz = 0, pz = 0;
     long long plane = 0, cells = 0;  // ny*pz, nx*ny*pz
-    int precision = 0, kernel = 0, tile_j = 4, tile_x = 0;
+    int precision = 0, kernel = 0, tile_j = 4, tile_x = 0, tile_z = 64;
     size_t esz = 8;
     bool sponge = false;
     double dt = 0;
     int nmat = 0;
     int dev = 0;
     hipStream_t stream = nullptr;
+    void* lvlbase[2] = {};
     void* lvl[2][3] = {};
     void* lvlo[2][3] = {};
     int cur = 0;

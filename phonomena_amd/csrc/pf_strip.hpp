@@ -23,6 +23,8 @@
 
 namespace pf {
 
+constexpr int KOUT = 62;  // output columns per 64-lane tile row
+
 struct StripMap {
     int nkt, nsg, nxc, xchunk;  // z tiles, strip groups (of W strips), x chunks, planes per chunk
 };

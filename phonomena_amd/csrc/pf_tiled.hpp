@@ -39,8 +39,6 @@
 
 namespace pf {
 
-constexpr int TK = 64;        // lanes along z
-constexpr int KOUT = TK - 2;  // output columns per tile
 constexpr int XT = 256;       // x-table entries: planes per chunk + 3
 
 struct TileMap {
@@ -74,12 +72,14 @@ PF_DEV double lane_above(double v) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Plane accesses go through buffer resources: one SGPR descriptor per array
-// (built once per block), the plane selected by the scalar byte offset
-// ip * plane * sizeof(R) and the cell by a per-lane 32-bit byte offset that
-// does not change along the march, so a load or store costs no per-lane
-// address arithmetic.  Arrays must be smaller than 4 GiB (the launcher
-// checks); every access made here lies inside its array.
+// Plane accesses go through buffer resources: one SGPR descriptor per time
+// level (whose arrays ux, uy, uz (, lo parts) lie back to back) and one for
+// the material indices, the array and plane selected by the scalar byte
+// offset q * cells * sizeof(R) + ip * plane * sizeof(R) and the cell by a
+// per-lane 32-bit byte offset that does not change along the march, so a
+// load or store costs no per-lane address arithmetic.  A level must be
+// smaller than 4 GiB (the launcher checks); every access made here lies
+// inside its level.
 typedef __amdgpu_buffer_rsrc_t Rsrc;
 // descriptor word 3 for untyped (raw) buffer access on gfx9-family GPUs
 constexpr int PF_RSRC_FLAGS = 0x00020000;
@@ -108,18 +108,18 @@ PF_DEV void bst(Rsrc r, unsigned off, unsigned soff, double v) {
 // A block-uniform value kept in a scalar register (plane offsets).
 PF_DEV unsigned uni(unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlane((int)v); }
 
-// Descriptors of every array the sweep streams.
+// Descriptors of the two levels and the index map; ab = bytes per array.
 struct Bufs {
-    Rsrc c0, c1, c2, n0, n1, n2, cl0, cl1, cl2, nl0, nl1, nl2, ids;
+    Rsrc cur, nxt, ids;
+    unsigned ab;
 };
-template <class R> PF_DEV Bufs make_bufs(const Mesh<R>& m) {
+template <class R, bool COMP> PF_DEV Bufs make_bufs(const Mesh<R>& m) {
     const long long n = (long long)m.nx * m.plane;
     Bufs b;
-    b.c0 = arr_rsrc(m.c0, n); b.c1 = arr_rsrc(m.c1, n); b.c2 = arr_rsrc(m.c2, n);
-    b.n0 = arr_rsrc(m.n0, n); b.n1 = arr_rsrc(m.n1, n); b.n2 = arr_rsrc(m.n2, n);
-    b.cl0 = arr_rsrc(m.cl0, n); b.cl1 = arr_rsrc(m.cl1, n); b.cl2 = arr_rsrc(m.cl2, n);
-    b.nl0 = arr_rsrc(m.nl0, n); b.nl1 = arr_rsrc(m.nl1, n); b.nl2 = arr_rsrc(m.nl2, n);
+    b.cur = arr_rsrc(m.c0, n * (COMP ? 6 : 3));
+    b.nxt = arr_rsrc(m.n0, n * (COMP ? 6 : 3));
     b.ids = arr_rsrc(m.ids, n);
+    b.ab = uni((unsigned)(n * (long long)sizeof(R)));
     return b;
 }
 
@@ -145,9 +145,9 @@ template <class R>
 PF_DEV Raw<R> load_raw(const Bufs& bf, int p, unsigned plane, unsigned boff, unsigned col) {
     Raw<R> r;
     const unsigned soff = uni((unsigned)p * plane * (unsigned)sizeof(R));
-    r.x = bld(bf.c0, boff, soff, R(0));
-    r.y = bld(bf.c1, boff, soff, R(0));
-    r.z = bld(bf.c2, boff, soff, R(0));
+    r.x = bld(bf.cur, boff, soff, R(0));
+    r.y = bld(bf.cur, boff, soff + bf.ab, R(0));
+    r.z = bld(bf.cur, boff, soff + 2 * bf.ab, R(0));
     r.id = __builtin_amdgcn_raw_buffer_load_b8(bf.ids, (int)col, (int)uni((unsigned)p * plane), 0);
     return r;
 }
@@ -175,16 +175,16 @@ template <class R, bool COMP>
 PF_DEV Old<R> load_old(const Bufs& bf, int i, unsigned plane, unsigned boff) {
     const unsigned soff = uni((unsigned)i * plane * (unsigned)sizeof(R));
     Old<R> o;
-    o.o0 = bld(bf.n0, boff, soff, R(0));
-    o.o1 = bld(bf.n1, boff, soff, R(0));
-    o.o2 = bld(bf.n2, boff, soff, R(0));
+    o.o0 = bld(bf.nxt, boff, soff, R(0));
+    o.o1 = bld(bf.nxt, boff, soff + bf.ab, R(0));
+    o.o2 = bld(bf.nxt, boff, soff + 2 * bf.ab, R(0));
     if constexpr (COMP) {
-        o.ol0 = bld(bf.nl0, boff, soff, R(0));
-        o.ol1 = bld(bf.nl1, boff, soff, R(0));
-        o.ol2 = bld(bf.nl2, boff, soff, R(0));
-        o.cl0 = bld(bf.cl0, boff, soff, R(0));
-        o.cl1 = bld(bf.cl1, boff, soff, R(0));
-        o.cl2 = bld(bf.cl2, boff, soff, R(0));
+        o.ol0 = bld(bf.nxt, boff, soff + 3 * bf.ab, R(0));
+        o.ol1 = bld(bf.nxt, boff, soff + 4 * bf.ab, R(0));
+        o.ol2 = bld(bf.nxt, boff, soff + 5 * bf.ab, R(0));
+        o.cl0 = bld(bf.cur, boff, soff + 3 * bf.ab, R(0));
+        o.cl1 = bld(bf.cur, boff, soff + 4 * bf.ab, R(0));
+        o.cl2 = bld(bf.cur, boff, soff + 5 * bf.ab, R(0));
     } else {
         o.ol0 = o.ol1 = o.ol2 = o.cl0 = o.cl1 = o.cl2 = R(0);
     }
@@ -206,71 +206,79 @@ template <class R> PF_DEV R sel(bool c, R a, R b) { return c ? a : b; }
 template <int Q, int MODE, class R>
 PF_DEV void put_tile(const Mesh<R>& m, const Bufs& bf, unsigned boff, unsigned soff, bool top, V<R> s0, V<R> cur,
                      V<R> above) {
-    const Rsrc nh = Q == 0 ? bf.n0 : (Q == 1 ? bf.n1 : bf.n2);
-    const Rsrc nl = Q == 0 ? bf.nl0 : (Q == 1 ? bf.nl1 : bf.nl2);
-    bst(nh, boff, soff, s0.h);
-    if constexpr (MODE == MODE_FP32C) bst(nl, boff, soff, s0.l);
+    const unsigned sh = soff + Q * bf.ab, sl = soff + (3 + Q) * bf.ab;
+    bst(bf.nxt, boff, sh, s0.h);
+    if constexpr (MODE == MODE_FP32C) bst(bf.nxt, boff, sl, s0.l);
     if (top) {
         const V<R> f = mur_map<MODE>(cur, m.mur[9 + Q], m.mur_lo[9 + Q], s0, above);
-        bst(nh, boff + (unsigned)sizeof(R), soff, f.h);
-        if constexpr (MODE == MODE_FP32C) bst(nl, boff + (unsigned)sizeof(R), soff, f.l);
+        bst(bf.nxt, boff + (unsigned)sizeof(R), sh, f.h);
+        if constexpr (MODE == MODE_FP32C) bst(bf.nxt, boff + (unsigned)sizeof(R), sl, f.l);
     }
 }
 
-template <class R, int MODE, bool SPONGE, int TJ>
-__global__ __launch_bounds__(TK * TJ, (MODE == MODE_FP32 ? 4 : 2)) void k_sweep_tiled(Mesh<R> m, TileMap tm) {
+// Shared memory of one block.  Material rows are addressed by byte offset
+// (id * row bytes), which is what the id registers and sid carry.
+template <class R, int TJ, int KZ>
+struct TiledLds {
+    R su[2][3][TJ][KZ];   // u planes, by plane parity
+    R sn2[2][TJ][KZ];     // T2 of a plane, by plane parity
+    R ss4[TJ][KZ];        // T4 of the current plane
+    R ss6[TJ][KZ];        // T6 of the current plane
+    uint16_t sid[2][TJ][KZ];
+    R smat[PF_TILED_MAX_MATERIALS * MAT_STRIDE];
+    XPlane<R> sxp[XT];    // planes ib-1 .. ie+1
+};
+
+template <class R> PF_DEV const R* mrow(const R* smat, int off) {
+    return (const R*)((const char*)smat + off);
+}
+template <class R> PF_DEV R mget(const R* smat, int off, int e) { return mrow(smat, off)[e]; }
+
+// DPP shifts filling 0 where the wavefront has no source lane (one
+// instruction; only halo lanes see the fill).
+PF_DEV int dpp_below0(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true); }
+PF_DEV int dpp_above0(int v) { return __builtin_amdgcn_mov_dpp(v, 0x130, 0xf, 0xf, true); }
+PF_DEV float below0(float v) { return __int_as_float(dpp_below0(__float_as_int(v))); }
+PF_DEV float above0(float v) { return __int_as_float(dpp_above0(__float_as_int(v))); }
+PF_DEV double below0(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_below0((int)(b & 0xffffffffLL)), hi = dpp_below0((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+PF_DEV double above0(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_above0((int)(b & 0xffffffffLL)), hi = dpp_above0((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// The march of one block over its tile and chunk.  FULL: every lane of the
+// tile, halos included, lies at 1 <= k <= nz-4 and 1 <= j <= ny-3, and the
+// chunk at 2 <= ib, ie <= nx-3, so no lane meets the free surface, a range
+// limit or a face, and every range select of the general code is the
+// identity; the FULL variant leaves them out (same bits).
+template <class R, int MODE, bool SPONGE, int TJ, int KZ, bool FULL>
+PF_DEV void tiled_march(const Mesh<R>& m, TiledLds<R, TJ, KZ>& L, int kl, int row, int k, int j, int ib, int ie,
+                        R w) {
     constexpr bool EXACT = MODE == MODE_FP64;
     constexpr bool COMP = MODE == MODE_FP32C;
-    __shared__ R su[2][3][TJ][TK];   // u planes, by plane parity
-    __shared__ R sn2[2][TJ][TK];     // T2 of a plane, by plane parity
-    __shared__ R ss4[TJ][TK];        // T4 of the current plane
-    __shared__ R ss6[TJ][TK];        // T6 of the current plane
-    __shared__ uint8_t sid[2][TJ][TK];
-    __shared__ R smat[PF_TILED_MAX_MATERIALS * MAT_STRIDE];
-    __shared__ XPlane<R> sxp[XT];    // planes ib-1 .. ie+1
-
-    const int lane = threadIdx.x, row = threadIdx.y;
-    const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-    const int kt = t % tm.nkt;
-    const int r_ = t / tm.nkt;
-    const int jt = r_ % tm.njt;
-    const int xc = r_ / tm.njt;
-    const int k = kt * KOUT - 1 + lane;
-    const int j = jt * (TJ - 2) - 1 + row;  // uniform over the wavefront
-    const int ib = xc * tm.xchunk;
-    const int ie = min(ib + tm.xchunk, m.nx);
-    const bool kval = k >= 0 && k < m.nz;
-    const bool jval = j >= 0 && j < m.ny;
-    const bool valid = kval && jval;
-    const bool writer = valid && lane >= 1 && lane <= TK - 2 && row >= 1 && row <= TJ - 2;
-    const long long tt = step_index(m);
-    if (tt >= m.nsteps) return;  // uniform over the grid
-    const R w = (R)m.wave[tt];
+    constexpr int RB = MAT_STRIDE * (int)sizeof(R);  // bytes per material row
     const int rp = row + 1 < TJ ? row + 1 : TJ - 1;
     const int rm = row > 0 ? row - 1 : 0;
-    const int tid = row * TK + lane;
-
-    for (int e = tid; e < m.nmat * MAT_STRIDE; e += TK * TJ) smat[e] = m.mat[e];
-    for (int e = tid; e < ie - ib + 3; e += TK * TJ) {
-        const int p = ib - 1 + e;
-        XPlane<R> x;
-        x.hs = (p >= 1 && p <= m.nx - 2) ? m.sx[p - 1] : R(1);
-        x.hf = (p >= 0 && p <= m.nx - 2) ? m.fx[p] : R(1);
-        x.dp = (SPONGE && p >= 0 && p < m.nx) ? m.dpx[p] : R(0);
-        x.pad = R(0);
-        sxp[e] = x;
-    }
 
     // Column constants.  Every k = 0 (free surface) variant of the
     // reference's expressions is the interior expression with other
     // spacings / operands, selected here once: the per-value formulas are
     // then evaluated without branches and give the same bits as the
     // separate surface expressions (x/1 = x and x - 0 = x exactly).
-    const bool ks = k >= 1;                    // interior in z
-    const bool jn = j >= 1 && j <= m.ny - 2;   // staggered y index valid
-    const bool jf = j >= 0 && j <= m.ny - 2;   // full y index valid
-    const bool kn = k >= 1 && k <= m.nz - 2;
-    const bool kf = k >= 0 && k <= m.nz - 2;
+    const bool kval = FULL || (k >= 0 && k < m.nz);
+    const bool jval = FULL || (j >= 0 && j < m.ny);
+    const bool valid = kval && jval;
+    const bool ks = FULL || k >= 1;                          // interior in z
+    const bool jn = FULL || (j >= 1 && j <= m.ny - 2);       // staggered y index valid
+    const bool jf = FULL || (j >= 0 && j <= m.ny - 2);       // full y index valid
+    const bool kn = FULL || (k >= 1 && k <= m.nz - 2);
+    const bool kf = FULL || (k >= 0 && k <= m.nz - 2);
+    const bool writer = valid && kl >= 1 && kl <= KZ - 2 && row >= 1 && row <= TJ - 2;
     const R hsy = jn ? m.sy[j - 1] : R(1);
     const R hfy = jf ? m.fy[j] : R(1);
     const R hsz = kn ? m.sz[k - 1] : R(1);
@@ -294,18 +302,18 @@ __global__ __launch_bounds__(TK * TJ, (MODE == MODE_FP32 ? 4 : 2)) void k_sweep_
     const bool wx = writer && jn && k <= m.nz - 2;
     const bool wy = writer && j >= 1 && j <= m.ny - 3 && k <= m.nz - 2;
     const bool wz = writer && jn && k <= m.nz - 3;
-    const bool top_xy = k == m.nz - 2;
-    const bool top_z = k == m.nz - 3;
+    const bool top_xy = !FULL && k == m.nz - 2;
+    const bool top_z = !FULL && k == m.nz - 3;
 
-    const Bufs bf = make_bufs(m);
+    const Bufs bf = make_bufs<R, COMP>(m);
     // prologue: plane ib-1 (warm-up, previous-plane buffer 1), then the
-    // loads in flight for the first plane of each parity
-    // Loads of planes the chunk does not need are clamped to plane ie (or
-    // nx-1), which the block reads anyway; planes outside the mesh are then
-    // zeroed by settle().
+    // loads in flight for the first plane of each parity.  Loads of planes
+    // the chunk does not need are clamped to plane ie (or nx-1), which the
+    // block reads anyway; planes outside the mesh are then zeroed by settle().
     const int pmax = ie < m.nx ? ie : m.nx - 1;
     Col<R> c0 = settle(load_raw<R>(bf, ib >= 1 ? ib - 1 : 0, plane, boff, col), valid && ib >= 1,
-                       ib - 1 == 0 && k == 0, w);
+                       !FULL && ib - 1 == 0 && k == 0, w);
+    c0.id *= RB;
     // issue order ra, rb, ob (kept by the scheduling barriers) matches the
     // steady state, so the loop needs no wait for more than it uses
     Raw<R> ra = load_raw<R>(bf, ib, plane, boff, col);
@@ -315,13 +323,14 @@ __global__ __launch_bounds__(TK * TJ, (MODE == MODE_FP32 ? 4 : 2)) void k_sweep_
     Old<R> ob = load_old<R, COMP>(bf, ib, plane, boff);
     __builtin_amdgcn_sched_barrier(0);
     Old<R> oa = {};  // the warm-up plane loads plane ib+1 into it
-    su[1][0][row][lane] = c0.x;
-    su[1][1][row][lane] = c0.y;
-    su[1][2][row][lane] = c0.z;
-    sid[1][row][lane] = (uint8_t)c0.id;
+    L.su[1][0][row][kl] = c0.x;
+    L.su[1][1][row][kl] = c0.y;
+    L.su[1][2][row][kl] = c0.z;
+    L.sid[1][row][kl] = (uint16_t)c0.id;
 
     R n1_0 = 0, n2_0 = 0, n3_0 = 0;  // N at plane i
     R s5_m = 0, s6_m = 0;            // T5, T6 at plane i-1
+    const R* smat = L.smat;
 
     // One plane i: H is the LDS parity of plane i+1, rin holds the raw
     // column of plane i+1 and receives plane i+3, oin holds the node values
@@ -329,95 +338,96 @@ __global__ __launch_bounds__(TK * TJ, (MODE == MODE_FP32 ? 4 : 2)) void k_sweep_
     auto plane_step = [&](int i, auto Hc, Raw<R>& rin, Old<R>& oin) {
         constexpr int H = decltype(Hc)::value;
         const int ip = i + 1;
-        const Col<R> c1 = settle(rin, valid && ip < m.nx, ip == 0 && k == 0, w);
-        su[H][0][row][lane] = c1.x;
-        su[H][1][row][lane] = c1.y;
-        su[H][2][row][lane] = c1.z;
-        sid[H][row][lane] = (uint8_t)c1.id;
+        Col<R> c1 = settle(rin, valid && (FULL || ip < m.nx), !FULL && ip == 0 && k == 0, w);
+        c1.id *= RB;
+        L.su[H][0][row][kl] = c1.x;
+        L.su[H][1][row][kl] = c1.y;
+        L.su[H][2][row][kl] = c1.z;
+        L.sid[H][row][kl] = (uint16_t)c1.id;
         __syncthreads();
         // issued after the barrier, so the registers of rin are free again
         rin = load_raw<R>(bf, ip + 2 <= pmax ? ip + 2 : pmax, plane, boff, col);
 
-        const XPlane<R> xq0 = sxp[i - ib + 1];  // x data of plane i
-        const XPlane<R> xq1 = sxp[ip - ib + 1];
-        const R uy1_jm = su[H][1][rm][lane];
-        const R uz1_km = lane_below(c1.z);
-        const R uy0_kp = lane_above(c0.y);
-        const R ux0_kp = lane_above(c0.x);
-        const R uz0_jp = su[1 - H][2][rp][lane];
-        const R ux0_jp = su[1 - H][0][rp][lane];
-        const int id_jp = sid[1 - H][rp][lane];     // (i, j+1, k)
-        const int id1_jp = sid[H][rp][lane];        // (i+1, j+1, k)
-        const int id_kp = dpp_from_above(c0.id);    // (i, j, k+1)
-        const int id_jkp = dpp_from_above(id_jp);   // (i, j+1, k+1)
-        const int id1_kp = dpp_from_above(c1.id);   // (i+1, j, k+1)
+        const XPlane<R> xq0 = L.sxp[i - ib + 1];  // x data of plane i
+        const XPlane<R> xq1 = L.sxp[ip - ib + 1];
+        const R uy1_jm = L.su[H][1][rm][kl];
+        const R uz1_km = below0(c1.z);
+        const R uy0_kp = above0(c0.y);
+        const R ux0_kp = above0(c0.x);
+        const R uz0_jp = L.su[1 - H][2][rp][kl];
+        const R ux0_jp = L.su[1 - H][0][rp][kl];
+        const int id_jp = L.sid[1 - H][rp][kl];   // (i, j+1, k)
+        const int id1_jp = L.sid[H][rp][kl];      // (i+1, j+1, k)
+        const int id_kp = dpp_above0(c0.id);      // (i, j, k+1)
+        const int id_jkp = dpp_above0(id_jp);     // (i, j+1, k+1)
+        const int id1_kp = dpp_above0(c1.id);     // (i+1, j, k+1)
 
         // normal stresses at plane ip (base_solver.py:329-351, 408-418)
         R n1_1, n2_1, n3_1;
         {
-            const R* c = smat + MAT_STRIDE * c1.id;
-            normal3<EXACT>(c, c1.x - c0.x, c1.y - uy1_jm, c1.z - sel(ks, uz1_km, R(0)), sel(ks, xq1.hs, m.sx0), hy_n,
-                           hz_n, n1_1, n2_1, n3_1);
-            const bool nv = ip >= 1 && ip <= m.nx - 2 && jn && kf;
-            n1_1 = sel(nv, n1_1, R(0));
-            n2_1 = sel(nv, n2_1, R(0));
-            n3_1 = sel(nv && ks, n3_1, R(0));
+            normal3<EXACT>(mrow(smat, c1.id), c1.x - c0.x, c1.y - uy1_jm, c1.z - sel(ks, uz1_km, R(0)),
+                           sel(ks, xq1.hs, m.sx0), hy_n, hz_n, n1_1, n2_1, n3_1);
+            if constexpr (!FULL) {
+                const bool nv = ip >= 1 && ip <= m.nx - 2 && jn && kf;
+                n1_1 = sel(nv, n1_1, R(0));
+                n2_1 = sel(nv, n2_1, R(0));
+                n3_1 = sel(nv && ks, n3_1, R(0));
+            }
         }
         // shear stresses at plane i (base_solver.py:353-372, 420-433)
-        const bool ii = i >= 0 && i <= m.nx - 2;
         const R fxi = sel(ks, xq0.hf, m.fz0);
-        R s4 = shear<EXACT>(smat[MAT_STRIDE * (ks ? id_jkp : id_jp) + M_C33], uy0_kp - c0.y, s4_h1,
-                            uz0_jp - c0.z, s4_h2);
-        R s5 = shear<EXACT>(smat[MAT_STRIDE * (ks ? id1_kp : c1.id) + M_C44], ux0_kp - c0.x, s5_h1,
-                            c1.z - c0.z, fxi);
-        R s6 = shear<EXACT>(smat[MAT_STRIDE * id1_jp + M_C55], ux0_jp - c0.x, s6_h1, c1.y - c0.y, fxi);
-        s4 = sel(ii && i >= 1 && jf && kf, s4, R(0));
-        s5 = sel(ii && jn && kf, s5, R(0));
-        s6 = sel(ii && jf && kf, s6, R(0));
-        sn2[H][row][lane] = n2_1;
-        ss4[row][lane] = s4;
-        ss6[row][lane] = s6;
+        R s4 = shear<EXACT>(mget(smat, ks ? id_jkp : id_jp, M_C33), uy0_kp - c0.y, s4_h1, uz0_jp - c0.z, s4_h2);
+        R s5 = shear<EXACT>(mget(smat, ks ? id1_kp : c1.id, M_C44), ux0_kp - c0.x, s5_h1, c1.z - c0.z, fxi);
+        R s6 = shear<EXACT>(mget(smat, id1_jp, M_C55), ux0_jp - c0.x, s6_h1, c1.y - c0.y, fxi);
+        if constexpr (!FULL) {
+            const bool ii = i >= 0 && i <= m.nx - 2;
+            s4 = sel(ii && i >= 1 && jf && kf, s4, R(0));
+            s5 = sel(ii && jn && kf, s5, R(0));
+            s6 = sel(ii && jf && kf, s6, R(0));
+        }
+        L.sn2[H][row][kl] = n2_1;
+        L.ss4[row][kl] = s4;
+        L.ss6[row][kl] = s6;
         __syncthreads();
 
-        const R n3_0_kp = lane_above(n3_0);
-        const R s4_km = lane_below(s4);
-        const R s5_km = lane_below(s5);
+        const R n3_0_kp = above0(n3_0);
+        const R s4_km = below0(s4);
+        const R s5_km = below0(s5);
         if (i >= ib && i < ie) {
-            const R n2_0_jp = sn2[1 - H][rp][lane];
-            const R s4_jm = ss4[rm][lane];
-            const R s6_jm = ss6[rm][lane];
+            const R n2_0_jp = L.sn2[1 - H][rp][kl];
+            const R s4_jm = L.ss4[rm][kl];
+            const R s6_jm = L.ss6[rm][kl];
             const R s = SPONGE ? (xq0.dp + dpyz) + dpz_k : R(0);
             const R gx = sel(ks, xq0.hs, m.sx0);
             const unsigned soff = uni((unsigned)i * plane * (unsigned)sizeof(R));
-            const R cl2 = (i == 0 && k == 0) ? R(0) : oin.cl2;
+            const R cl2 = (!FULL && i == 0 && k == 0) ? R(0) : oin.cl2;
+            const bool wall = !FULL && i == 0;  // uy, uz on the x = 0 wall stay 0
             // ux (base_solver.py:441-447, 495-501)
             {
                 const R dv = div3<EXACT>(n1_1 - n1_0, sel(ks, xq0.hf, m.fx0), s6 - s6_jm, hy_n,
                                          s5 - sel(ks, s5_km, R(0)), hz_n);
                 const V<R> cur = {c0.x, oin.cl0};
-                const V<R> s0 =
-                    leap_from<MODE, SPONGE>(cur, oin.o0, oin.ol0, smat[MAT_STRIDE * c1.id + M_COEF], dv, s);
-                const V<R> above = {ux0_kp, COMP ? lane_above(oin.cl0) : R(0)};
-                if (wx && i <= m.nx - 3) put_tile<0, MODE>(m, bf, boff, soff, top_xy, s0, cur, above);
+                const V<R> s0 = leap_from<MODE, SPONGE>(cur, oin.o0, oin.ol0, mget(smat, c1.id, M_COEF), dv, s);
+                const V<R> above = {ux0_kp, COMP ? above0(oin.cl0) : R(0)};
+                if (wx && (FULL || i <= m.nx - 3)) put_tile<0, MODE>(m, bf, boff, soff, top_xy, s0, cur, above);
             }
             // uy (base_solver.py:449-455, 503-509)
             {
                 const R dv = div3<EXACT>(s6 - s6_m, gx, n2_0_jp - n2_0, uy_g2, s4 - sel(ks, s4_km, R(0)), hz_n);
                 const V<R> cur = {c0.y, oin.cl1};
-                V<R> s0 = leap_from<MODE, SPONGE>(cur, oin.o1, oin.ol1, smat[MAT_STRIDE * id_jp + M_COEF], dv, s);
-                if (i == 0) s0 = {R(0), R(0)};
-                const V<R> above = {uy0_kp, COMP ? lane_above(oin.cl1) : R(0)};
-                if (wy && i <= m.nx - 2) put_tile<1, MODE>(m, bf, boff, soff, top_xy, s0, cur, above);
+                V<R> s0 = leap_from<MODE, SPONGE>(cur, oin.o1, oin.ol1, mget(smat, id_jp, M_COEF), dv, s);
+                if (wall) s0 = {R(0), R(0)};
+                const V<R> above = {uy0_kp, COMP ? above0(oin.cl1) : R(0)};
+                if (wy && (FULL || i <= m.nx - 2)) put_tile<1, MODE>(m, bf, boff, soff, top_xy, s0, cur, above);
             }
             // uz (base_solver.py:457-463, 511-517)
             {
                 const R dv = div3<EXACT>(s5 - s5_m, gx, s4 - s4_jm, hy_n, n3_0_kp - n3_0, uz_g3);
                 const V<R> cur = {c0.z, cl2};
-                V<R> s0 = leap_from<MODE, SPONGE>(cur, oin.o2, oin.ol2,
-                                                  smat[MAT_STRIDE * (ks ? id_kp : c0.id) + M_COEF], dv, s);
-                if (i == 0) s0 = {R(0), R(0)};
-                const V<R> above = {lane_above(c0.z), COMP ? lane_above(cl2) : R(0)};
-                if (wz && i <= m.nx - 2) put_tile<2, MODE>(m, bf, boff, soff, top_z, s0, cur, above);
+                V<R> s0 = leap_from<MODE, SPONGE>(cur, oin.o2, oin.ol2, mget(smat, ks ? id_kp : c0.id, M_COEF), dv, s);
+                if (wall) s0 = {R(0), R(0)};
+                const V<R> above = {above0(c0.z), COMP ? above0(cl2) : R(0)};
+                if (wz && (FULL || i <= m.nx - 2)) put_tile<2, MODE>(m, bf, boff, soff, top_z, s0, cur, above);
             }
         }
         oin = load_old<R, COMP>(bf, i + 2 < ie ? i + 2 : ie - 1, plane, boff);
@@ -439,9 +449,50 @@ __global__ __launch_bounds__(TK * TJ, (MODE == MODE_FP32 ? 4 : 2)) void k_sweep_
     }
 }
 
-inline TileMap tile_map(const pf_ctx* c, int tj, int xchunk) {
+// KZ lanes along z per tile row (64: one row per wavefront; 32: two rows
+// per wavefront, lanes 0-31 and 32-63), TJ rows along y per block.
+template <class R, int MODE, bool SPONGE, int TJ, int KZ>
+__global__ __launch_bounds__(TJ * KZ, (MODE == MODE_FP32 ? 4 : 2)) void k_sweep_tiled(Mesh<R> m, TileMap tm) {
+    __shared__ TiledLds<R, TJ, KZ> L;
+    const int kl = threadIdx.x % KZ;
+    const int row = threadIdx.y * (64 / KZ) + threadIdx.x / KZ;
+    const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int kt = t % tm.nkt;
+    const int r_ = t / tm.nkt;
+    const int jt = r_ % tm.njt;
+    const int xc = r_ / tm.njt;
+    const int k = kt * (KZ - 2) - 1 + kl;
+    const int j = jt * (TJ - 2) - 1 + row;
+    const int ib = xc * tm.xchunk;
+    const int ie = min(ib + tm.xchunk, m.nx);
+    const long long tt = step_index(m);
+    if (tt >= m.nsteps) return;  // uniform over the grid
+    const R w = (R)m.wave[tt];
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+
+    for (int e = tid; e < m.nmat * MAT_STRIDE; e += TJ * KZ) L.smat[e] = m.mat[e];
+    for (int e = tid; e < ie - ib + 3; e += TJ * KZ) {
+        const int p = ib - 1 + e;
+        XPlane<R> x;
+        x.hs = (p >= 1 && p <= m.nx - 2) ? m.sx[p - 1] : R(1);
+        x.hf = (p >= 0 && p <= m.nx - 2) ? m.fx[p] : R(1);
+        x.dp = (SPONGE && p >= 0 && p < m.nx) ? m.dpx[p] : R(0);
+        x.pad = R(0);
+        L.sxp[e] = x;
+    }
+    // block-uniform: the whole tile and chunk are clear of every boundary
+    const int k_lo = kt * (KZ - 2) - 1, k_hi = k_lo + KZ - 1;
+    const int j_lo = jt * (TJ - 2) - 1, j_hi = j_lo + TJ - 1;
+    const bool full = k_lo >= 1 && k_hi <= m.nz - 4 && j_lo >= 1 && j_hi <= m.ny - 3 && ib >= 2 && ie <= m.nx - 3;
+    if (full)
+        tiled_march<R, MODE, SPONGE, TJ, KZ, true>(m, L, kl, row, k, j, ib, ie, w);
+    else
+        tiled_march<R, MODE, SPONGE, TJ, KZ, false>(m, L, kl, row, k, j, ib, ie, w);
+}
+
+inline TileMap tile_map(const pf_ctx* c, int tj, int kz, int xchunk) {
     TileMap tm;
-    tm.nkt = (c->nz + KOUT - 1) / KOUT;
+    tm.nkt = (c->nz + (kz - 2) - 1) / (kz - 2);
     tm.njt = (c->ny + (tj - 2) - 1) / (tj - 2);
     if (xchunk <= 0) {
         // aim for >= ~2048 blocks (8 per CU) with chunks of at least 16 planes
@@ -458,24 +509,43 @@ inline TileMap tile_map(const pf_ctx* c, int tj, int xchunk) {
     return tm;
 }
 
-template <class R, int MODE, bool SPONGE>
-int launch_tiled_s(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
-    const int tj = c->tile_j;
-    if ((unsigned long long)c->cells * sizeof(R) >= (1ULL << 32))
-        return fail(PF_ERR_ARG, "tiled sweep: field arrays must be < 4 GiB (use the naive kernel)");
-    if (c->tile_x > XT - 3) return fail(PF_ERR_ARG, "tile_x must be <= %d (got %d)", XT - 3, c->tile_x);
-    TileMap tm = tile_map(c, tj, c->tile_x);
+// Lane-slot count of a tiling (lanes x planes), used to pick the z layout.
+inline double tiled_slots(const pf_ctx* c, int tj, int kz) {
+    const TileMap tm = tile_map(c, tj, kz, c->tile_x);
+    return (double)tm.nkt * kz * tm.njt * tj * tm.nxc * (tm.xchunk + 2);
+}
+
+template <class R, int MODE, bool SPONGE, int TJ, int KZ>
+int launch_tiled_k(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
+    TileMap tm = tile_map(c, TJ, KZ, c->tile_x);
     const long long nb = (long long)tm.nkt * tm.njt * tm.nxc;
     if (nb > 0x7fffffffLL) return fail(PF_ERR_ARG, "too many tiles");
-    const dim3 grd((unsigned)nb);
-    switch (tj) {
-        case 4: hipLaunchKernelGGL((k_sweep_tiled<R, MODE, SPONGE, 4>), grd, dim3(TK, 4), 0, s, m, tm); break;
-        case 8: hipLaunchKernelGGL((k_sweep_tiled<R, MODE, SPONGE, 8>), grd, dim3(TK, 8), 0, s, m, tm); break;
-        case 16: hipLaunchKernelGGL((k_sweep_tiled<R, MODE, SPONGE, 16>), grd, dim3(TK, 16), 0, s, m, tm); break;
-        default: return fail(PF_ERR_ARG, "tile_j must be 4, 8 or 16 (got %d)", tj);
-    }
+    hipLaunchKernelGGL((k_sweep_tiled<R, MODE, SPONGE, TJ, KZ>), dim3((unsigned)nb), dim3(64, TJ * KZ / 64), 0, s, m,
+                       tm);
     HIPCHK(hipGetLastError());
     return PF_OK;
+}
+
+template <class R, int MODE, bool SPONGE>
+int launch_tiled_s(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
+    if ((unsigned long long)c->cells * sizeof(R) * (MODE == MODE_FP32C ? 6 : 3) >= (1ULL << 32))
+        return fail(PF_ERR_ARG, "tiled sweep: a time level must be < 4 GiB (use the naive kernel)");
+    if (c->tile_x > XT - 3) return fail(PF_ERR_ARG, "tile_x must be <= %d (got %d)", XT - 3, c->tile_x);
+    const int tj = c->tile_j, kz = c->tile_z;
+    if (kz == 64) {
+        switch (tj) {
+            case 4: return launch_tiled_k<R, MODE, SPONGE, 4, 64>(c, m, s);
+            case 8: return launch_tiled_k<R, MODE, SPONGE, 8, 64>(c, m, s);
+            case 16: return launch_tiled_k<R, MODE, SPONGE, 16, 64>(c, m, s);
+        }
+    } else if (kz == 32) {
+        switch (tj) {
+            case 8: return launch_tiled_k<R, MODE, SPONGE, 8, 32>(c, m, s);
+            case 16: return launch_tiled_k<R, MODE, SPONGE, 16, 32>(c, m, s);
+            case 32: return launch_tiled_k<R, MODE, SPONGE, 32, 32>(c, m, s);
+        }
+    }
+    return fail(PF_ERR_ARG, "tiled sweep: tile_j %d with tile_z %d is not a built shape", tj, kz);
 }
 
 template <class R, int MODE>

@@ -47,7 +47,7 @@ class PfDesc(ctypes.Structure):
         ("sponge_x", ctypes.c_void_p), ("sponge_y", ctypes.c_void_p), ("sponge_z", ctypes.c_void_p),
         ("wave", ctypes.c_void_p), ("wave_len", ctypes.c_int64),
         ("probe", ctypes.c_int32 * 6), ("record_every", ctypes.c_int32), ("probe_capacity", ctypes.c_int32),
-        ("tile_j", ctypes.c_int32), ("tile_x", ctypes.c_int32),
+        ("tile_j", ctypes.c_int32), ("tile_x", ctypes.c_int32), ("tile_z", ctypes.c_int32),
     ]
 
 
@@ -160,7 +160,7 @@ class MeshSpec:
 class Context:
     """One mesh resident on the GPU (a pf_ctx)."""
 
-    def __init__(self, spec, precision="fp64", kernel="auto", tile_j=0, tile_x=0):
+    def __init__(self, spec, precision="fp64", kernel="auto", tile_j=0, tile_x=0, tile_z=0):
         self.lib = load()
         self.spec = spec
         self.precision = precision
@@ -191,7 +191,7 @@ class Context:
                 d.probe[n] = v
             d.record_every = spec.record_every
             d.probe_capacity = spec.probe_capacity
-        d.tile_j, d.tile_x = int(tile_j), int(tile_x)
+        d.tile_j, d.tile_x, d.tile_z = int(tile_j), int(tile_x), int(tile_z)
         h = ctypes.c_void_p()
         check(self.lib.pf_create(ctypes.byref(d), ctypes.byref(h)))
         self.handle = h

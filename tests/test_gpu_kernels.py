@@ -11,8 +11,8 @@ pytestmark = pytest.mark.gpu
 FIELDS = ["ux", "uy", "uz", "ux_old", "uy_old", "uz_old"]
 
 
-def _fields(spec, precision, kernel, tile_j=0, tile_x=0, steps=None):
-    ctx = fdtd.Context(spec, precision=precision, kernel=kernel, tile_j=tile_j, tile_x=tile_x)
+def _fields(spec, precision, kernel, tile_j=0, tile_x=0, steps=None, tile_z=0):
+    ctx = fdtd.Context(spec, precision=precision, kernel=kernel, tile_j=tile_j, tile_x=tile_x, tile_z=tile_z)
     ctx.step(spec.wave.size if steps is None else steps)
     ctx.sync()
     out = {f: ctx.get(f) for f in FIELDS}
@@ -49,15 +49,40 @@ def test_fp64_tiled_bitwise_vs_reference():
             assert np.array_equal(out[f], d[f]), (name, f)
 
 
-@pytest.mark.parametrize("tile_j,tile_x", [(4, 0), (4, 2), (8, 1), (8, 7), (16, 0), (16, 33), (16, 70)])
+@pytest.mark.parametrize("tile_j,tile_x,tile_z", [(4, 0, 64), (4, 2, 64), (8, 1, 64), (8, 7, 64), (16, 0, 64),
+                                                 (16, 33, 64), (16, 70, 64), (8, 3, 32), (16, 0, 32), (32, 0, 32),
+                                                 (32, 9, 32)])
 @pytest.mark.parametrize("precision", ["fp32", "fp32c"])
-def test_tiled_shapes_on_crystal(tile_j, tile_x, precision):
+def test_tiled_shapes_on_crystal(tile_j, tile_x, tile_z, precision):
     # z extent not a multiple of the 62 output lanes, y not of the row count
     x, y, z, ids, rows, dt, wave = uniform_crystal(
         (70, 45, 131), 1.0, incl=(19300.0, 2.0e11, 1.6e11, 14, 5), steps=120, seed=5)
     spec = spec_from(x, y, z, ids, rows, dt, wave, probe=(0, 3, 24, 1, 2, 23), every=7)
     a = _fields(spec, precision, "naive")
-    b = _fields(spec, precision, "tiled", tile_j=tile_j, tile_x=tile_x)
+    b = _fields(spec, precision, "tiled", tile_j=tile_j, tile_x=tile_x, tile_z=tile_z)
+    for f in a:
+        assert np.array_equal(a[f], b[f]), f
+
+
+@pytest.mark.parametrize("tile_j,tile_x,tile_z", [(8, 0, 64), (16, 3, 32), (8, 5, 32)])
+def test_tiled_fp64_shapes_on_crystal(tile_j, tile_x, tile_z):
+    x, y, z, ids, rows, dt, wave = uniform_crystal(
+        (70, 45, 131), 1.0, incl=(19300.0, 2.0e11, 1.6e11, 14, 5), steps=60, seed=5)
+    spec = spec_from(x, y, z, ids, rows, dt, wave)
+    a = _fields(spec, "fp64", "naive")
+    b = _fields(spec, "fp64", "tiled", tile_j=tile_j, tile_x=tile_x, tile_z=tile_z)
+    for f in a:
+        assert np.array_equal(a[f], b[f]), f
+
+
+@pytest.mark.parametrize("tile_j,tile_z", [(32, 32), (16, 64)])
+def test_tiled_interior_tiles_on_larger_mesh(tile_j, tile_z):
+    # big enough that most blocks take the boundary-free (interior) path
+    x, y, z, ids, rows, dt, wave = uniform_crystal(
+        (120, 100, 140), 1.0, incl=(19300.0, 2.0e11, 1.6e11, 20, 7), steps=40, seed=11)
+    spec = spec_from(x, y, z, ids, rows, dt, wave)
+    a = _fields(spec, "fp32", "naive")
+    b = _fields(spec, "fp32", "tiled", tile_j=tile_j, tile_z=tile_z)
     for f in a:
         assert np.array_equal(a[f], b[f]), f
 
