@@ -7,14 +7,22 @@ phononic crystal (513x513x257 nodes; square lattice a = 24 cells, r = 8
 cells, +-0.5 cell jitter, seed 3; matrix rho 2700, lambda 5.1e10, mu 2.6e10;
 tungsten-like inclusions rho 19300, lambda 2.0e11, mu 1.6e11; Courant 0.5;
 Gaussian-modulated 2 MHz wall source for dx = 5e-5 m, i.e. 100 cycles per
-unit time in the reference's unit-cell coordinates).  A step is one time step
-over the whole mesh; cell-updates = nodes x steps (the reference counts
-nodes, tests/test_speed.py:52).  Fields start at zero and the source drives
-them; inputs are synthetic and resident in HBM before the timed region.
+unit time in the reference's unit-cell coordinates; 16-cell absorbing layer;
+surface u_z probes on a 64x64 subgrid every 10 steps).  Workload "c2" is the
+same recipe on the 256x256x128-cell crystal (a = 16, r = 5, seed 2).  A step
+is one time step over the whole mesh; cell-updates = nodes x steps (the
+reference counts nodes, tests/test_speed.py:52).  Fields start at zero and
+the source drives them; inputs are synthetic and resident in HBM before the
+timed region.
 
 Prints one JSON line.  The roofline entry uses the sweep kernel's mean
 duration from HIP events recorded on its own stream during the timed
-steps, and the algorithmic bytes of one launch.
+steps, and the algorithmic bytes of one launch: 48 / 72 / 96 B per
+cell-update for FP32 / FP32 compensated / FP64 (u^n and u^{n-1} read,
+u^{n+1} written, three per-cell coefficients read; 12 values, plus the three
+compensation terms read and written in the compensated mode).  The sweep
+itself reads a 1-byte material index instead of three coefficients, so its
+own minimum is 37 / 73 / 73 B (``kernel_bytes_per_node``).
 """
 import argparse
 import json
@@ -28,13 +36,22 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic HBM bytes per node per step of the sweep: read u^n (3) and
-# u^{n-1} (3), write u^{n+1} (3), read the material index (1 byte)
-BYTES_PER_NODE = {"fp32": 9 * 4 + 1, "fp32c": 18 * 4 + 1, "fp64": 9 * 8 + 1}
+# algorithmic HBM bytes per cell-update (12 values: u^n, u^{n-1}, u^{n+1},
+# 3 coefficients; + 3 compensation terms read and written when compensated)
+BYTES_PER_NODE = {"fp32": 12 * 4, "fp32c": 18 * 4, "fp64": 12 * 8}
+# what this sweep itself must move: 9 field values (18 hi/lo in FP32C) and
+# the 1-byte material index
+KERNEL_BYTES_PER_NODE = {"fp32": 9 * 4 + 1, "fp32c": 18 * 4 + 1, "fp64": 9 * 8 + 1}
+WORKLOADS = {
+    "plain": dict(cells=(512, 512, 256), a=24, r=8, seed=3),
+    "c2": dict(cells=(256, 256, 128), a=16, r=5, seed=2),
+}
+SPONGE = {"width": 16}
+PROBE = {"count": 64}
 
 
 def crystal_spec(cells, a, r, seed, steps, kind="gauss", f0=2e6, dx_phys=5e-5, probe=None,
-                 inclusion=None):
+                 inclusion=None, sponge=None):
     """Crystal in the reference's unit-cell coordinates; the physical cell
     size dx_phys maps the source frequency to f0 * dx_phys per unit time."""
     from phonomena_amd import problem
@@ -46,7 +63,7 @@ def crystal_spec(cells, a, r, seed, steps, kind="gauss", f0=2e6, dx_phys=5e-5, p
     b.m = m
     b.cfg["wave"], b.cfg["wave_args"] = kind, {"f": f0 * dx_phys}
     wave = b.source_table(steps)
-    spec = problem.mesh_spec(g, m, wave, {"probe": probe, "record_every": 10})
+    spec = problem.mesh_spec(g, m, wave, {"probe": probe, "record_every": 10, "sponge": sponge})
     return spec, g, m
 
 
@@ -56,9 +73,9 @@ def cpu_baseline(budget_s=20.0):
     crystal recipe; 1 core (NumPy ufuncs are single-threaded)."""
     from oracle import reference_math as rm
     cells = (128, 128, 64)
-    spec, g, m = crystal_spec(cells, 24, 8, 3, 64)
+    spec, g, m = crystal_spec(cells, 24, 8, 3, 64, sponge=SPONGE)
     C, P = m.C, m.P
-    pb = rm.Problem(g.x, g.y, g.z, C, P, m.dt, spec.wave)
+    pb = rm.Problem(g.x, g.y, g.z, C, P, m.dt, spec.wave, sponge=spec.sponge)
     s = rm.State(*pb.shape)
     rm.step(s, pb, 0)  # warm
     n = 0
@@ -69,9 +86,18 @@ def cpu_baseline(budget_s=20.0):
     dt = time.perf_counter() - t0
     nodes = int(np.prod(pb.shape))
     return {"value": nodes * n / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": "oracle (NumPy restatement of base_solver.py's step, dense C[...,6,6] layout) "
-                      "on a %dx%dx%d-cell block of the same crystal recipe, %d steps, %.1f s; host has "
+            "sample": "oracle (NumPy restatement of base_solver.py's step, dense C[...,6,6] layout, "
+                      "16-cell absorbing layer) on a %dx%dx%d-cell block of the same crystal recipe, "
+                      "%d steps, %.1f s; host has "
                       "%d CPUs (%d usable)" % (cells + (n, dt, os.cpu_count(), len(os.sched_getaffinity(0))))}
+
+
+def stream_peaks(nodes):
+    """Practical HBM bandwidth on this device for a plain copy and for the
+    sweep's mix (6 field arrays read, 3 written), arrays of the mesh's size."""
+    from phonomena_amd import fdtd
+    n = nodes // 4 * 4
+    return {"copy_1r1w": fdtd.stream_probe(1, 1, n, 10)[0], "mix_6r3w": fdtd.stream_probe(6, 3, n, 10)[0]}
 
 
 def traffic_from_profiles(workload, precision):
@@ -89,7 +115,9 @@ def traffic_from_profiles(workload, precision):
 def bench_plain(args):
     from phonomena_amd import fdtd
     steps = args.warmup + args.steps
-    spec, g, m = crystal_spec((512, 512, 256), 24, 8, 3, steps)
+    wl = WORKLOADS[args.workload]
+    spec, g, m = crystal_spec(wl["cells"], wl["a"], wl["r"], wl["seed"], steps,
+                              sponge=None if args.no_sponge else SPONGE, probe=PROBE)
     nodes = spec.cells
     ctx = fdtd.Context(spec, precision=args.precision, kernel=args.kernel, tile_j=args.tile_j,
                        tile_x=args.tile_x, tile_z=args.tile_z)
@@ -115,15 +143,20 @@ def bench_plain(args):
         "vs_baseline": None,
         "dtype": {"fp32": "f32", "fp32c": "f32 (hi+lo compensated)", "fp64": "f64"}[args.precision],
         "data": "synthetic",
-        "config": {"workload": "plain %s sweep, 512x512x256-cell phononic crystal (513x513x257 nodes), "
-                               "lattice a=24 r=8 cells, seed 3, gauss 2 MHz source" % args.precision,
+        "config": {"workload": "plain %s sweep, %dx%dx%d-cell phononic crystal (%dx%dx%d nodes), "
+                               "lattice a=%d r=%d cells, seed %d, gauss 2 MHz source, %s, 64x64 surface "
+                               "probes every 10 steps" % ((args.precision,) + wl["cells"] + spec.nodes +
+                                                          (wl["a"], wl["r"], wl["seed"],
+                                                           "no absorbing layer" if args.no_sponge
+                                                           else "16-cell absorbing layer")),
                    "nodes": nodes, "kernel": args.kernel, "tile_j": args.tile_j, "tile_x": args.tile_x,
                    "tile_z": args.tile_z,
                    "fill_fraction": float(m.ids.mean())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_from_profiles("plain", args.precision),
-                     "kernel": "sweep", "bytes_per_node": bpn, "sweep_ms": sweep_ms, "face_ms": face_ms},
+                     "traffic": traffic_from_profiles(args.workload, args.precision),
+                     "kernel": "sweep", "bytes_per_node": bpn, "kernel_bytes_per_node": KERNEL_BYTES_PER_NODE[args.precision],
+                     "sweep_ms": sweep_ms, "face_ms": face_ms},
     }
     ctx.close()
     return rec
@@ -134,19 +167,23 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="plain", choices=["plain"])
+    ap.add_argument("--workload", default="plain", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-sponge", action="store_true", help="Mur faces only, no absorbing layer")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32c", "fp64"])
     ap.add_argument("--kernel", default="auto", choices=["auto", "naive", "tiled", "strip"])
     ap.add_argument("--tile-j", type=int, default=0, help="tiled sweep rows per block (4..32)")
     ap.add_argument("--tile-z", type=int, default=0, help="tiled sweep lanes along z per row (32, 64)")
     ap.add_argument("--tile-x", type=int, default=0, help="tiled sweep x planes per block")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-stream", action="store_true", help="skip the streaming-bandwidth probe")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design "
               "note (slab decomposition along x), not implemented", file=sys.stderr)
         sys.exit(2)
     rec = bench_plain(args)
+    if not args.no_stream:
+        rec["roofline"]["stream_peak_gbs"] = stream_peaks(rec["config"]["nodes"])
     rec["cpu_baseline"] = None if args.no_cpu else cpu_baseline()
     print(json.dumps(rec), flush=True)
 

@@ -141,6 +141,16 @@ int pf_batch_sync(pf_batch *b);
 int pf_batch_set_tiles(pf_batch *b, int32_t tile_j, int32_t tile_x, int32_t use_graph);
 int pf_batch_destroy(pf_batch *b);
 
+/*
+ * Diagnostics (no counterpart in the reference).  Streams `reads` input and
+ * `writes` output float arrays of n elements each through the device `reps`
+ * times with a plain coalesced kernel and returns the achieved GB/s and the
+ * mean ms per pass: the practical HBM bandwidth of that mix on this device,
+ * and a kernel of known traffic for calibrating hardware counters.  Built
+ * mixes (reads/writes): 1/1, 2/1, 6/3, 7/3, 1/0.  Allocates its own buffers.
+ */
+int pf_stream_probe(int32_t reads, int32_t writes, int64_t n, int32_t reps, double *gbs, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -90,6 +90,7 @@ def load(path=None):
         "pf_batch_sync": ([V], ctypes.c_int),
         "pf_batch_set_tiles": ([V, I32, I32, I32], ctypes.c_int),
         "pf_batch_destroy": ([V], ctypes.c_int),
+        "pf_stream_probe": ([I32, I32, I64, I32, P(D), P(D)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -115,6 +116,13 @@ def device_info():
     gib = ctypes.c_double()
     check(load().pf_device_info(name, 256, ctypes.byref(cu), ctypes.byref(gib)))
     return {"name": name.value.decode(), "cu": cu.value, "hbm_gib": gib.value}
+
+
+def stream_probe(reads, writes, n, reps=20):
+    """Practical HBM bandwidth of a read/write mix: (GB/s, ms per pass)."""
+    g, t = ctypes.c_double(), ctypes.c_double()
+    check(load().pf_stream_probe(int(reads), int(writes), int(n), int(reps), ctypes.byref(g), ctypes.byref(t)))
+    return g.value, t.value
 
 
 def _c(a, dtype):
