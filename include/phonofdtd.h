@@ -129,16 +129,33 @@ int pf_energy(pf_ctx *ctx, double *out);
 int pf_profile(pf_ctx *ctx, int64_t nsteps, double *sweep_ms, double *face_ms);
 
 /*
- * Batches: many contexts of one precision stepped together, one launch per
- * stage over a combined tile work-list, the step loop captured in a HIP
- * graph.  Each member stops at its own step budget (wave_len).
+ * Batches (no counterpart in the reference, whose solver steps one mesh):
+ * many contexts of one precision stepped together.  A batched step is four
+ * launches whatever the member count: one persistent sweep over a combined
+ * tile work-list of all members, then all members' boundary faces, probes
+ * and step counters; the step loop is captured in a HIP graph.  Each member
+ * stops at its own step budget (wave_len) while the others go on, and its
+ * fields are bit-identical to stepping it alone.  While a context is a
+ * member, pf_step/pf_set_field/pf_profile on it fail (PF_ERR_STATE) and
+ * pf_get_field/pf_get_probes/pf_energy wait for the batch's work.
  */
 int pf_batch_create(int32_t precision, pf_batch **out);
-int pf_batch_add(pf_batch *b, pf_ctx *ctx);
+int pf_batch_add(pf_batch *b, pf_ctx *ctx);      /* same precision, <= 16
+                                                    materials, all members
+                                                    with or all without an
+                                                    absorbing layer          */
 int pf_batch_remove(pf_batch *b, pf_ctx *ctx);
-int pf_batch_step(pf_batch *b, int64_t nsteps);
+int pf_batch_step(pf_batch *b, int64_t nsteps);  /* asynchronous            */
 int pf_batch_sync(pf_batch *b);
-int pf_batch_set_tiles(pf_batch *b, int32_t tile_j, int32_t tile_x, int32_t use_graph);
+/* Tile shape of the batched sweep: rows (tile_j) x lanes along z (tile_z:
+   16x64, 8x64, 32x32, 16x32, 8x32; 0 = default), planes per x chunk
+   (tile_x, 0 = 32), and whether steps are replayed from a HIP graph. */
+int pf_batch_set_tiles(pf_batch *b, int32_t tile_j, int32_t tile_z, int32_t tile_x, int32_t use_graph);
+/* Autotuner: times every candidate shape over trial_steps real steps (all
+   shapes give the same bits, so the members simply advance) and keeps the
+   fastest; returns it and its time per batched step.                     */
+int pf_batch_autotune(pf_batch *b, int32_t trial_steps, int32_t *tile_j, int32_t *tile_z, int32_t *tile_x,
+                      double *ms_per_step);
 int pf_batch_destroy(pf_batch *b);
 
 /*

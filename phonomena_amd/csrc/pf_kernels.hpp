@@ -450,19 +450,14 @@ __global__ __launch_bounds__(256) void k_sweep_naive(Mesh<R> m) {
 }
 
 // ------------------------------------------------------------ probes ----
-// Surface u_z of the newest level at (i0 + si*a, j0 + sj*b, 0) into frame
-// slot (*frame % cap); one block, so the counter update is ordered after
+// One probe frame (surface u_z subgrid, hi + lo) into slot `frame % cap` of
+// the ring, after step tt, when (tt + 1) is a multiple of `every`.  Called
+// by all threads of one block; thread 0 advances the frame counter after
 // every read by the block barrier.
 template <class R>
-__global__ __launch_bounds__(256) void k_probe(const R* __restrict__ uz, const R* __restrict__ uzl,
-                                               int plane_ny_pz_dummy, long long plane, int pz,
-                                               int i0, int si, int ni, int j0, int sj, int nj,
-                                               double* __restrict__ ring, int cap,
-                                               long long* __restrict__ frame,
-                                               const long long* __restrict__ step_base,
-                                               long long step_off, int every, long long nsteps) {
-    (void)plane_ny_pz_dummy;
-    long long tt = (step_base ? *step_base : 0) + step_off;  // step just completed
+PF_DEV void probe_record(const R* __restrict__ uz, const R* __restrict__ uzl, long long plane, int pz,
+                         int i0, int si, int ni, int j0, int sj, int nj, double* __restrict__ ring, int cap,
+                         long long* __restrict__ frame, long long tt, int every, long long nsteps) {
     if (tt >= nsteps || (tt + 1) % every != 0) return;
     long long f = *frame;
     double* out = ring + (f % cap) * (long long)ni * nj;
@@ -475,6 +470,19 @@ __global__ __launch_bounds__(256) void k_probe(const R* __restrict__ uz, const R
     }
     __syncthreads();
     if (threadIdx.x == 0) *frame = f + 1;
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_probe(const R* __restrict__ uz, const R* __restrict__ uzl,
+                                               int plane_ny_pz_dummy, long long plane, int pz,
+                                               int i0, int si, int ni, int j0, int sj, int nj,
+                                               double* __restrict__ ring, int cap,
+                                               long long* __restrict__ frame,
+                                               const long long* __restrict__ step_base,
+                                               long long step_off, int every, long long nsteps) {
+    (void)plane_ny_pz_dummy;
+    const long long tt = (step_base ? *step_base : 0) + step_off;  // step just completed
+    probe_record<R>(uz, uzl, plane, pz, i0, si, ni, j0, sj, nj, ring, cap, frame, tt, every, nsteps);
 }
 
 // Tiny kernel: advance a device step counter after a captured chunk.

@@ -449,14 +449,13 @@ PF_DEV void tiled_march(const Mesh<R>& m, TiledLds<R, TJ, KZ>& L, int kl, int ro
     }
 }
 
-// KZ lanes along z per tile row (64: one row per wavefront; 32: two rows
-// per wavefront, lanes 0-31 and 32-63), TJ rows along y per block.
+// One tile of a sweep: tile index t (after any remapping) of mesh m under
+// tile map tm, LDS L.  Every thread of the block calls it; it ends without
+// a barrier, so a caller that reuses L for another tile syncs first.
 template <class R, int MODE, bool SPONGE, int TJ, int KZ>
-__global__ __launch_bounds__(TJ * KZ, (MODE == MODE_FP32 ? 4 : 2)) void k_sweep_tiled(Mesh<R> m, TileMap tm) {
-    __shared__ TiledLds<R, TJ, KZ> L;
+PF_DEV void tiled_tile(const Mesh<R>& m, const TileMap& tm, int t, TiledLds<R, TJ, KZ>& L) {
     const int kl = threadIdx.x % KZ;
     const int row = threadIdx.y * (64 / KZ) + threadIdx.x / KZ;
-    const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int kt = t % tm.nkt;
     const int r_ = t / tm.nkt;
     const int jt = r_ % tm.njt;
@@ -488,6 +487,14 @@ __global__ __launch_bounds__(TJ * KZ, (MODE == MODE_FP32 ? 4 : 2)) void k_sweep_
         tiled_march<R, MODE, SPONGE, TJ, KZ, true>(m, L, kl, row, k, j, ib, ie, w);
     else
         tiled_march<R, MODE, SPONGE, TJ, KZ, false>(m, L, kl, row, k, j, ib, ie, w);
+}
+
+// KZ lanes along z per tile row (64: one row per wavefront; 32: two rows
+// per wavefront, lanes 0-31 and 32-63), TJ rows along y per block.
+template <class R, int MODE, bool SPONGE, int TJ, int KZ>
+__global__ __launch_bounds__(TJ * KZ, (MODE == MODE_FP32 ? 4 : 2)) void k_sweep_tiled(Mesh<R> m, TileMap tm) {
+    __shared__ TiledLds<R, TJ, KZ> L;
+    tiled_tile<R, MODE, SPONGE, TJ, KZ>(m, tm, xcd_remap((int)blockIdx.x, (int)gridDim.x), L);
 }
 
 inline TileMap tile_map(const pf_ctx* c, int tj, int kz, int xchunk) {

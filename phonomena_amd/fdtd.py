@@ -88,7 +88,8 @@ def load(path=None):
         "pf_batch_remove": ([V, V], ctypes.c_int),
         "pf_batch_step": ([V, I64], ctypes.c_int),
         "pf_batch_sync": ([V], ctypes.c_int),
-        "pf_batch_set_tiles": ([V, I32, I32, I32], ctypes.c_int),
+        "pf_batch_set_tiles": ([V, I32, I32, I32, I32], ctypes.c_int),
+        "pf_batch_autotune": ([V, I32, P(I32), P(I32), P(I32), P(D)], ctypes.c_int),
         "pf_batch_destroy": ([V], ctypes.c_int),
         "pf_stream_probe": ([I32, I32, I64, I32, P(D), P(D)], ctypes.c_int),
     }
@@ -263,3 +264,56 @@ class Context:
         a, b = ctypes.c_double(), ctypes.c_double()
         check(self.lib.pf_profile(self.handle, int(n), ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+
+class Batch:
+    """Many contexts of one precision stepped together (a pf_batch).
+
+    Members keep their own step budgets; each member's fields and probes are
+    bit-identical to stepping it alone.  While a context is a member, step it
+    through the batch only.
+    """
+
+    def __init__(self, precision="fp32"):
+        self.lib = load()
+        self.precision = precision
+        h = ctypes.c_void_p()
+        check(self.lib.pf_batch_create(PRECISION[precision], ctypes.byref(h)))
+        self.handle = h
+        self.members = []
+
+    def add(self, ctx):
+        check(self.lib.pf_batch_add(self.handle, ctx.handle))
+        self.members.append(ctx)
+
+    def remove(self, ctx):
+        check(self.lib.pf_batch_remove(self.handle, ctx.handle))
+        self.members.remove(ctx)
+
+    def step(self, n):
+        check(self.lib.pf_batch_step(self.handle, int(n)))
+
+    def sync(self):
+        check(self.lib.pf_batch_sync(self.handle))
+
+    def set_tiles(self, tile_j=0, tile_z=0, tile_x=0, graph=True):
+        check(self.lib.pf_batch_set_tiles(self.handle, int(tile_j), int(tile_z), int(tile_x), 1 if graph else 0))
+
+    def autotune(self, trial_steps=4):
+        """Time every candidate tile shape on real steps; keep the fastest."""
+        j, z, x, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
+        check(self.lib.pf_batch_autotune(self.handle, int(trial_steps), ctypes.byref(j), ctypes.byref(z),
+                                         ctypes.byref(x), ctypes.byref(ms)))
+        return {"tile_j": j.value, "tile_z": z.value, "tile_x": x.value, "ms_per_step": ms.value}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            check(self.lib.pf_batch_destroy(self.handle))
+            self.handle = None
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
