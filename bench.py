@@ -50,21 +50,10 @@ SPONGE = {"width": 16}
 PROBE = {"count": 64}
 
 
-def crystal_spec(cells, a, r, seed, steps, kind="gauss", f0=2e6, dx_phys=5e-5, probe=None,
-                 inclusion=None, sponge=None):
-    """Crystal in the reference's unit-cell coordinates; the physical cell
-    size dx_phys maps the source frequency to f0 * dx_phys per unit time."""
-    from phonomena_amd import problem
-    from phonomena_amd.simulation import lattice
-    from phonomena_amd.simulation.base_solver import BaseSolver
-
-    g, m = lattice.crystal(cells, a, r, seed, inclusion=inclusion or lattice.TUNGSTEN)
-    b = BaseSolver(None)
-    b.m = m
-    b.cfg["wave"], b.cfg["wave_args"] = kind, {"f": f0 * dx_phys}
-    wave = b.source_table(steps)
-    spec = problem.mesh_spec(g, m, wave, {"probe": probe, "record_every": 10, "sponge": sponge})
-    return spec, g, m
+def crystal_spec(cells, a, r, seed, steps, probe=None, sponge=None):
+    """Crystal in the reference's unit-cell coordinates (phonomena_amd.sweep)."""
+    from phonomena_amd import sweep
+    return sweep.crystal_spec(cells, a, r, seed, steps, probe=probe, sponge=sponge)
 
 
 def cpu_baseline(budget_s=20.0):
@@ -162,6 +151,62 @@ def bench_plain(args):
     return rec
 
 
+def bench_batched(steps, precision="fp32", n=64, seed=64, trial_steps=2):
+    """The 64-crystal mix: the plain single-instance loop (one context at a
+    time, pf_step) against one batch of all of them (autotuned, graph-
+    replayed), same steps and precision; accuracy of the batched fields
+    against an FP64 run of the same meshes (FP64 here is bit-identical to
+    the reference's arithmetic).  Set-up (meshing, upload) is not timed."""
+    from phonomena_amd import fdtd, sweep
+    jobs = sweep.mix(n, seed, steps)
+    specs = [sweep.job_spec(j, probe={"count": 32}, every=10) for j in jobs]
+    nodes = sum(sp.cells for sp in specs)
+    # plain loop
+    t_plain = 0.0
+    for sp in specs:
+        c = fdtd.Context(sp, precision=precision)
+        t0 = time.perf_counter()
+        c.step(steps)
+        c.sync()
+        t_plain += time.perf_counter() - t0
+        c.close()
+    # batched
+    ctxs = [fdtd.Context(sp, precision=precision) for sp in specs]
+    b = fdtd.Batch(precision)
+    for c in ctxs:
+        b.add(c)
+    t0 = time.perf_counter()
+    tuned = b.autotune(trial_steps) if trial_steps > 0 else None
+    done = ctxs[0].steps_done
+    b.step(steps - done)
+    b.sync()
+    t_batch = time.perf_counter() - t0
+    # FP64 run of the same meshes, as one FP64 batch
+    c64 = [fdtd.Context(sp, precision="fp64") for sp in specs]
+    b64 = fdtd.Batch("fp64")
+    for c in c64:
+        b64.add(c)
+    b64.step(steps)
+    b64.sync()
+    worst = 0.0
+    for c, r in zip(ctxs, c64):
+        for f in ("ux", "uy", "uz"):
+            worst = max(worst, sweep.rel_l2(c.get(f), r.get(f)))
+    b.close()
+    b64.close()
+    for c in ctxs + c64:
+        c.close()
+    v_plain = nodes * steps / t_plain
+    v_batch = nodes * steps / t_batch
+    return {"workload": "%d-crystal mix, seed %d: 48x48x24..192x192x96 cells, lattice a 8-24 cells, r/a "
+                        "0.15-0.45, void-like or tungsten-like, gauss 2 MHz source, 32x32 probes every 10 "
+                        "steps, %d steps each, %s" % (n, seed, steps, precision),
+            "meshes": n, "nodes_total": nodes, "steps": steps,
+            "value": v_batch, "unit": "cell-updates/s", "plain_loop": v_plain,
+            "speedup_vs_plain": v_batch / v_plain, "seconds_batched": t_batch, "seconds_plain": t_plain,
+            "rel_l2_max_vs_fp64": worst, "autotuned": tuned}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,6 +221,8 @@ def main():
     ap.add_argument("--tile-x", type=int, default=0, help="tiled sweep x planes per block")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-stream", action="store_true", help="skip the streaming-bandwidth probe")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 64-crystal leg")
+    ap.add_argument("--batch-steps", type=int, default=2000, help="steps per crystal of the batched leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design "
@@ -184,6 +231,8 @@ def main():
     rec = bench_plain(args)
     if not args.no_stream:
         rec["roofline"]["stream_peak_gbs"] = stream_peaks(rec["config"]["nodes"])
+    if not args.no_batch:
+        rec["batched"] = bench_batched(args.batch_steps, precision=args.precision)
     rec["cpu_baseline"] = None if args.no_cpu else cpu_baseline()
     print(json.dumps(rec), flush=True)
 

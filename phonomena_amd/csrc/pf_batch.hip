@@ -54,15 +54,41 @@ struct BMember {
     long long budget;       // source-table length
 };
 
+// The member table is read through the constant address space: its loads
+// are scalar (one per block, into scalar registers), as a kernel argument's
+// are.  Through a plain pointer every use of a mesh field inside the march
+// became a vector load with a full wait.
 template <class R>
-PF_DEV const Mesh<R>& current_view(const BMember<R>& b) {
-    const long long tt = *b.step;
-    return b.view[b.cur0 ^ (int)((tt - b.t0) & 1)];
-}
+using CMember = const __attribute__((address_space(4))) BMember<R>;
 
 struct WorkItem {
     int member, tile;
 };
+typedef const __attribute__((address_space(4))) WorkItem CItem;
+
+// A copy of *p read word by word through the constant address space
+// (block-uniform p), i.e. with scalar loads.
+template <class T>
+PF_DEV T load_const(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "word-sized struct");
+    typedef const __attribute__((address_space(4))) unsigned int CWord;
+    const CWord* w = (const CWord*)p;
+    T v;
+    unsigned int* d = (unsigned int*)&v;
+#pragma unroll
+    for (unsigned k = 0; k < sizeof(T) / 4; k++) d[k] = w[k];
+    return v;
+}
+
+// Mesh view of member i at its current step (the step counter is written
+// only between launches, so one read per block is current for the launch).
+template <class R>
+PF_DEV Mesh<R> current_view(const BMember<R>* mem, int i) {
+    CMember<R>* cm = (CMember<R>*)mem;
+    const long long tt = *cm[i].step;
+    const int p = cm[i].cur0 ^ (int)((tt - cm[i].t0) & 1);
+    return load_const(&mem[i].view[p]);
+}
 
 template <class R, int MODE, bool SPONGE, int TJ, int KZ>
 __global__ __launch_bounds__(TJ * KZ, (MODE == MODE_FP32 ? 4 : 2)) void k_batch_sweep(
@@ -80,11 +106,14 @@ __global__ __launch_bounds__(TJ * KZ, (MODE == MODE_FP32 ? 4 : 2)) void k_batch_
         __syncthreads();  // the previous tile's LDS reads and s_item are done
         if (tid == 0) s_item = qbeg[x] + atomicAdd(c, 1);
         __syncthreads();
-        const int it = s_item;
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
         if (it >= end) break;  // uniform: every wave of the block leaves
-        const WorkItem wk = items[it];
-        const BMember<R>& b = mem[wk.member];
-        tiled_tile<R, MODE, SPONGE, TJ, KZ>(current_view(b), b.tm, wk.tile, L);
+        CItem* ci = (CItem*)items;
+        const int mi = ci[it].member, tile = ci[it].tile;
+        CMember<R>* cm = (CMember<R>*)mem;
+        const TileMap tm = {cm[mi].tm.nkt, cm[mi].tm.njt, cm[mi].tm.nxc, cm[mi].tm.xchunk};
+        const Mesh<R> m = current_view(mem, mi);
+        tiled_tile<R, MODE, SPONGE, TJ, KZ>(m, tm, tile, L);
     }
 }
 
@@ -99,18 +128,24 @@ __global__ __launch_bounds__(256) void k_batch_faces(const BMember<R>* __restric
         const int mid = (lo + hi + 1) / 2;
         if (fblk[mid] <= g) lo = mid; else hi = mid - 1;
     }
-    const BMember<R>& b = mem[lo];
-    const long long t = (g - fblk[lo]) * 256 + threadIdx.x;
-    if (t < b.faces) faces_one<R, MODE>(current_view(b), t);
+    const int mi = __builtin_amdgcn_readfirstlane(lo);
+    CMember<R>* cm = (CMember<R>*)mem;
+    const long long t = (g - fblk[mi]) * 256 + threadIdx.x;
+    if (t < cm[mi].faces) {
+        const Mesh<R> m = current_view(mem, mi);
+        faces_one<R, MODE>(m, t);
+    }
 }
 
 template <class R>
 __global__ __launch_bounds__(256) void k_batch_probe(const BMember<R>* __restrict__ mem) {
-    const BMember<R>& b = mem[blockIdx.x];
-    if (b.probe[2] <= 0) return;
-    const Mesh<R>& m = current_view(b);
-    probe_record<R>(m.n2, m.nl2, m.plane, m.pz, b.probe[0], b.probe[1], b.probe[2], b.probe[3], b.probe[4],
-                    b.probe[5], b.ring, b.cap, b.frame, *b.step, b.every, b.budget);
+    CMember<R>* cm = (CMember<R>*)mem;
+    const int i = (int)blockIdx.x;
+    if (cm[i].probe[2] <= 0) return;
+    const Mesh<R> m = current_view(mem, i);
+    probe_record<R>(m.n2, m.nl2, m.plane, m.pz, cm[i].probe[0], cm[i].probe[1], cm[i].probe[2], cm[i].probe[3],
+                    cm[i].probe[4], cm[i].probe[5], cm[i].ring, cm[i].cap, cm[i].frame, *cm[i].step, cm[i].every,
+                    cm[i].budget);
 }
 
 static __global__ __launch_bounds__(256) void k_batch_advance(long long* const* __restrict__ steps,
