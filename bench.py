@@ -151,60 +151,81 @@ def bench_plain(args):
     return rec
 
 
-def bench_batched(steps, precision="fp32", n=64, seed=64, trial_steps=2):
+def bench_batched(steps, precisions=("fp32c", "fp32"), n=64, seed=64, trial_steps=2):
     """The 64-crystal mix: the plain single-instance loop (one context at a
     time, pf_step) against one batch of all of them (autotuned, graph-
-    replayed), same steps and precision; accuracy of the batched fields
-    against an FP64 run of the same meshes (FP64 here is bit-identical to
-    the reference's arithmetic).  Set-up (meshing, upload) is not timed."""
+    replayed), same steps and precision, and the accuracy of the batched
+    fields against an FP64 run of the same meshes (the GPU FP64 mode is
+    bit-identical to the reference's arithmetic).  Set-up (meshing, upload)
+    is not timed.  The first precision is the one reported as ``value``."""
     from phonomena_amd import fdtd, sweep
     jobs = sweep.mix(n, seed, steps)
     specs = [sweep.job_spec(j, probe={"count": 32}, every=10) for j in jobs]
     nodes = sum(sp.cells for sp in specs)
-    # plain loop
-    t_plain = 0.0
-    for sp in specs:
-        c = fdtd.Context(sp, precision=precision)
-        t0 = time.perf_counter()
-        c.step(steps)
-        c.sync()
-        t_plain += time.perf_counter() - t0
-        c.close()
-    # batched
-    ctxs = [fdtd.Context(sp, precision=precision) for sp in specs]
-    b = fdtd.Batch(precision)
-    for c in ctxs:
-        b.add(c)
-    t0 = time.perf_counter()
-    tuned = b.autotune(trial_steps) if trial_steps > 0 else None
-    done = ctxs[0].steps_done
-    b.step(steps - done)
-    b.sync()
-    t_batch = time.perf_counter() - t0
-    # FP64 run of the same meshes, as one FP64 batch
+    # FP64 fields of the same meshes, as one FP64 batch
     c64 = [fdtd.Context(sp, precision="fp64") for sp in specs]
     b64 = fdtd.Batch("fp64")
     for c in c64:
         b64.add(c)
     b64.step(steps)
     b64.sync()
-    worst = 0.0
-    for c, r in zip(ctxs, c64):
-        for f in ("ux", "uy", "uz"):
-            worst = max(worst, sweep.rel_l2(c.get(f), r.get(f)))
-    b.close()
+    ref = [{f: c.get(f) for f in ("ux", "uy", "uz")} for c in c64]
     b64.close()
-    for c in ctxs + c64:
+    for c in c64:
         c.close()
-    v_plain = nodes * steps / t_plain
-    v_batch = nodes * steps / t_batch
+    out = {}
+    for prec in precisions:
+        t_plain = 0.0
+        for sp in specs:
+            c = fdtd.Context(sp, precision=prec)
+            t0 = time.perf_counter()
+            c.step(steps)
+            c.sync()
+            t_plain += time.perf_counter() - t0
+            c.close()
+        ctxs = [fdtd.Context(sp, precision=prec) for sp in specs]
+        b = fdtd.Batch(prec)
+        for c in ctxs:
+            b.add(c)
+        t0 = time.perf_counter()
+        tuned = b.autotune(trial_steps) if trial_steps > 0 else None
+        b.step(steps - ctxs[0].steps_done)
+        b.sync()
+        t_batch = time.perf_counter() - t0
+        worst = max(sweep.rel_l2(c.get(f), r[f]) for c, r in zip(ctxs, ref) for f in ("ux", "uy", "uz"))
+        b.close()
+        for c in ctxs:
+            c.close()
+        out[prec] = {"value": nodes * steps / t_batch, "plain_loop": nodes * steps / t_plain,
+                     "speedup_vs_plain": t_plain / t_batch, "seconds_batched": t_batch,
+                     "seconds_plain": t_plain, "rel_l2_max_vs_fp64": worst, "autotuned": tuned}
+    head = out[precisions[0]]
     return {"workload": "%d-crystal mix, seed %d: 48x48x24..192x192x96 cells, lattice a 8-24 cells, r/a "
                         "0.15-0.45, void-like or tungsten-like, gauss 2 MHz source, 32x32 probes every 10 "
-                        "steps, %d steps each, %s" % (n, seed, steps, precision),
-            "meshes": n, "nodes_total": nodes, "steps": steps,
-            "value": v_batch, "unit": "cell-updates/s", "plain_loop": v_plain,
-            "speedup_vs_plain": v_batch / v_plain, "seconds_batched": t_batch, "seconds_plain": t_plain,
-            "rel_l2_max_vs_fp64": worst, "autotuned": tuned}
+                        "steps, %d steps each" % (n, seed, steps),
+            "meshes": n, "nodes_total": nodes, "steps": steps, "precision": precisions[0],
+            "value": head["value"], "unit": "cell-updates/s", "plain_loop": head["plain_loop"],
+            "speedup_vs_plain": head["speedup_vs_plain"], "rel_l2_max_vs_fp64": head["rel_l2_max_vs_fp64"],
+            "by_precision": out}
+
+
+def bench_served(n_jobs, capacity_jobs_per_s, precision="fp32c", load=0.7, seed=5, chunk=200):
+    """Served design-sweep stream: Poisson arrivals at `load` x the batched
+    capacity (jobs/s), grids U[32^3 .. 256x256x128] cells, U{500..8000}
+    steps, lattice parameters as in the batched mix, 200 GB HBM budget,
+    32x32 probe frames streamed back every 10 steps."""
+    from phonomena_amd import serve
+    jobs = serve.stream_jobs(n_jobs, seed)
+    rate = load * capacity_jobs_per_s
+    arrivals = serve.poisson_arrivals(n_jobs, rate, seed)
+    srv = serve.Server(precision=precision, hbm_budget=200e9, chunk=chunk, probe={"count": 32}, record_every=10)
+    results, stats = srv.run(jobs, arrivals)
+    stats.update({"workload": "served stream of %d jobs, seed %d: Poisson arrivals at %.2f jobs/s (%.1f x the "
+                              "batched capacity), 32^3..256x256x128-cell crystals, 500-8000 steps, %s, "
+                              "200 GB HBM budget, 32x32 probes every 10 steps streamed back"
+                              % (n_jobs, seed, rate, load, precision),
+                  "arrival_rate_jobs_per_s": rate})
+    return stats
 
 
 def main():
@@ -223,6 +244,8 @@ def main():
     ap.add_argument("--no-stream", action="store_true", help="skip the streaming-bandwidth probe")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 64-crystal leg")
     ap.add_argument("--batch-steps", type=int, default=2000, help="steps per crystal of the batched leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
+    ap.add_argument("--serve-jobs", type=int, default=48, help="jobs in the served stream")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design "
@@ -232,7 +255,13 @@ def main():
     if not args.no_stream:
         rec["roofline"]["stream_peak_gbs"] = stream_peaks(rec["config"]["nodes"])
     if not args.no_batch:
-        rec["batched"] = bench_batched(args.batch_steps, precision=args.precision)
+        rec["batched"] = bench_batched(args.batch_steps)
+        if not args.no_serve:
+            from phonomena_amd import serve, sweep
+            # capacity in jobs/s: batched throughput over the mean work of a served job
+            jobs = serve.stream_jobs(256, 5)
+            mean_work = np.mean([np.prod([c + 1 for c in j["cells"]]) * j["steps"] for j in jobs])
+            rec["served"] = bench_served(args.serve_jobs, rec["batched"]["value"] / mean_work)
     rec["cpu_baseline"] = None if args.no_cpu else cpu_baseline()
     print(json.dumps(rec), flush=True)
 

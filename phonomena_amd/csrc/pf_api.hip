@@ -216,11 +216,16 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
             if (c->precision == PF_FP32C) c->lvlo[l][q] = b + (3 + q) * bytes;
         }
     }
-    // material index map, padded like the fields
+    // material index map, in the field layout (rows unpadded: one copy; a
+    // row-by-row 2-D copy took ~0.3 s per million nodes)
     HIPCHK(hipMalloc((void**)&c->ids, (size_t)c->cells));
-    HIPCHK(hipMemset(c->ids, 0, (size_t)c->cells));
-    HIPCHK(hipMemcpy2D(c->ids, (size_t)c->pz, d->ids, (size_t)d->nz, (size_t)d->nz,
-                       (size_t)d->nx * d->ny, hipMemcpyHostToDevice));
+    if (c->pz == d->nz) {
+        HIPCHK(hipMemcpy(c->ids, d->ids, (size_t)c->cells, hipMemcpyHostToDevice));
+    } else {
+        HIPCHK(hipMemset(c->ids, 0, (size_t)c->cells));
+        HIPCHK(hipMemcpy2D(c->ids, (size_t)c->pz, d->ids, (size_t)d->nz, (size_t)d->nz,
+                           (size_t)d->nx * d->ny, hipMemcpyHostToDevice));
+    }
 
     int rc = c->esz == 8 ? create_tables<double>(c, d) : create_tables<float>(c, d);
     if (rc) return rc;
