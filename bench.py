@@ -209,6 +209,38 @@ def bench_batched(steps, precisions=("fp32c", "fp32"), n=64, seed=64, trial_step
             "by_precision": out}
 
 
+def bench_precision(steps, workload="plain"):
+    """The precision case: the crystal run for `steps` steps in FP32
+    compensated (and plain FP32) against a GPU FP64 run of the same mesh and
+    source (FP64 here is bit-identical to the reference's arithmetic);
+    per-field rel-L2 at the final step and the drift of the FP64 energy
+    audit, |E - E_64| / E_64."""
+    from phonomena_amd import fdtd, sweep
+    wl = WORKLOADS[workload]
+    spec, _, _ = crystal_spec(wl["cells"], wl["a"], wl["r"], wl["seed"], steps, sponge=SPONGE, probe=PROBE)
+    out = {"workload": "%dx%dx%d-cell crystal (a=%d r=%d seed %d, C2 recipe), %d steps" % (
+        wl["cells"] + (wl["a"], wl["r"], wl["seed"], steps)), "steps": steps}
+    ref = None
+    for prec in ("fp64", "fp32c", "fp32"):
+        c = fdtd.Context(spec, precision=prec)
+        t0 = time.perf_counter()
+        c.step(steps)
+        c.sync()
+        secs = time.perf_counter() - t0
+        e = c.energy()
+        f = {q: c.get(q) for q in ("ux", "uy", "uz")}
+        c.close()
+        if ref is None:
+            ref = (f, e)
+            out["fp64"] = {"energy": e, "seconds": secs}
+            continue
+        out[prec] = {"rel_l2": {q: sweep.rel_l2(f[q], ref[0][q]) for q in f},
+                     "energy": e, "energy_drift_vs_fp64": abs(e - ref[1]) / abs(ref[1]) if ref[1] else None,
+                     "seconds": secs}
+        del f
+    return out
+
+
 def bench_served(n_jobs, capacity_jobs_per_s, precision="fp32c", load=0.7, seed=5, chunk=200):
     """Served design-sweep stream: Poisson arrivals at `load` x the batched
     capacity (jobs/s), grids U[32^3 .. 256x256x128] cells, U{500..8000}
@@ -245,6 +277,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 64-crystal leg")
     ap.add_argument("--batch-steps", type=int, default=2000, help="steps per crystal of the batched leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
+    ap.add_argument("--precision-steps", type=int, default=10000,
+                    help="steps of the FP32C/FP32 vs FP64 precision case (0 = skip)")
     ap.add_argument("--serve-jobs", type=int, default=48, help="jobs in the served stream")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -262,6 +296,8 @@ def main():
             jobs = serve.stream_jobs(256, 5)
             mean_work = np.mean([np.prod([c + 1 for c in j["cells"]]) * j["steps"] for j in jobs])
             rec["served"] = bench_served(args.serve_jobs, rec["batched"]["value"] / mean_work)
+    if args.precision_steps > 0:
+        rec["precision_case"] = bench_precision(args.precision_steps)
     rec["cpu_baseline"] = None if args.no_cpu else cpu_baseline()
     print(json.dumps(rec), flush=True)
 
