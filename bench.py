@@ -21,8 +21,9 @@ steps, and the algorithmic bytes of one launch: 48 / 72 / 96 B per
 cell-update for FP32 / FP32 compensated / FP64 (u^n and u^{n-1} read,
 u^{n+1} written, three per-cell coefficients read; 12 values, plus the three
 compensation terms read and written in the compensated mode).  The sweep
-itself reads a 1-byte material index instead of three coefficients, so its
-own minimum is 37 / 73 / 73 B (``kernel_bytes_per_node``).
+itself reads a 1-byte material index instead of three coefficients and
+stores compensation terms in 16 bits, so its own minimum is 37 / 55 / 73 B
+(``kernel_bytes_per_node``).
 """
 import argparse
 import json
@@ -39,9 +40,9 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic HBM bytes per cell-update (12 values: u^n, u^{n-1}, u^{n+1},
 # 3 coefficients; + 3 compensation terms read and written when compensated)
 BYTES_PER_NODE = {"fp32": 12 * 4, "fp32c": 18 * 4, "fp64": 12 * 8}
-# what this sweep itself must move: 9 field values (18 hi/lo in FP32C) and
-# the 1-byte material index
-KERNEL_BYTES_PER_NODE = {"fp32": 9 * 4 + 1, "fp32c": 18 * 4 + 1, "fp64": 9 * 8 + 1}
+# what this sweep itself must move: 9 field values (FP32C: 4-byte hi and
+# 2-byte lo each) and the 1-byte material index
+KERNEL_BYTES_PER_NODE = {"fp32": 9 * 4 + 1, "fp32c": 9 * (4 + 2) + 1, "fp64": 9 * 8 + 1}
 WORKLOADS = {
     "plain": dict(cells=(512, 512, 256), a=24, r=8, seed=3),
     "c2": dict(cells=(256, 256, 128), a=16, r=5, seed=2),

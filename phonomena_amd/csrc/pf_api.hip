@@ -161,8 +161,7 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
     c->precision = d->precision;
     const bool few_mats = d->n_materials <= PF_TILED_MAX_MATERIALS;
     // the tiled sweep addresses a whole level with 32-bit offsets
-    const bool small = (unsigned long long)c->cells * (d->precision == PF_FP64 ? 8 : 4) *
-                           (d->precision == PF_FP32C ? 6 : 3) < (1ULL << 32);
+    const bool small = level_bytes(c->cells, d->precision) < (1ULL << 32);
     c->kernel = d->kernel == PF_KERNEL_AUTO ? (few_mats && small ? PF_KERNEL_TILED : PF_KERNEL_NAIVE) : d->kernel;
     if ((c->kernel == PF_KERNEL_TILED || c->kernel == PF_KERNEL_STRIP) && !few_mats)
         return fail(PF_ERR_ARG, "the tiled and strip sweeps hold at most %d materials (got %d)",
@@ -203,17 +202,18 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
     HIPCHK(hipGetDevice(&c->dev));
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
 
-    // fields: two levels x three components (+ lo parts in compensated mode),
-    // each level one allocation
+    // fields: two levels x three components (+ 16-bit lo parts in
+    // compensated mode), each level one allocation
     const size_t bytes = (size_t)c->cells * c->esz;
-    const int per = c->precision == PF_FP32C ? 6 : 3;
+    const size_t lbytes = (size_t)c->cells * sizeof(lo_t);
+    const size_t lvl_bytes = level_bytes(c->cells, c->precision);
     for (int l = 0; l < 2; l++) {
-        HIPCHK(hipMalloc(&c->lvlbase[l], bytes * per));
-        HIPCHK(hipMemset(c->lvlbase[l], 0, bytes * per));
+        HIPCHK(hipMalloc(&c->lvlbase[l], lvl_bytes));
+        HIPCHK(hipMemset(c->lvlbase[l], 0, lvl_bytes));
         char* b = (char*)c->lvlbase[l];
         for (int q = 0; q < 3; q++) {
             c->lvl[l][q] = b + q * bytes;
-            if (c->precision == PF_FP32C) c->lvlo[l][q] = b + (3 + q) * bytes;
+            if (c->precision == PF_FP32C) c->lvlo[l][q] = b + 3 * bytes + q * lbytes;
         }
     }
     // material index map, in the field layout (rows unpadded: one copy; a
@@ -315,8 +315,8 @@ Mesh<R> pf::make_view(const pf_ctx* c, int cur) {
     const int nxt = 1 - cur;
     m.c0 = (const R*)c->lvl[cur][0]; m.c1 = (const R*)c->lvl[cur][1]; m.c2 = (const R*)c->lvl[cur][2];
     m.n0 = (R*)c->lvl[nxt][0]; m.n1 = (R*)c->lvl[nxt][1]; m.n2 = (R*)c->lvl[nxt][2];
-    m.cl0 = (const R*)c->lvlo[cur][0]; m.cl1 = (const R*)c->lvlo[cur][1]; m.cl2 = (const R*)c->lvlo[cur][2];
-    m.nl0 = (R*)c->lvlo[nxt][0]; m.nl1 = (R*)c->lvlo[nxt][1]; m.nl2 = (R*)c->lvlo[nxt][2];
+    m.cl0 = (const lo_t*)c->lvlo[cur][0]; m.cl1 = (const lo_t*)c->lvlo[cur][1]; m.cl2 = (const lo_t*)c->lvlo[cur][2];
+    m.nl0 = (lo_t*)c->lvlo[nxt][0]; m.nl1 = (lo_t*)c->lvlo[nxt][1]; m.nl2 = (lo_t*)c->lvlo[nxt][2];
     m.ids = c->ids;
     m.mat = (const R*)c->mat;
     m.fx = (const R*)c->fx; m.fy = (const R*)c->fy; m.fz = (const R*)c->fz;
@@ -370,7 +370,7 @@ template <class R>
 static int launch_probe_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
     if (c->probe[2] <= 0) return PF_OK;
     // newest level after this step = the n* buffers of the view
-    hipLaunchKernelGGL((k_probe<R>), dim3(1), dim3(256), 0, s, (const R*)m.n2, (const R*)m.nl2, 0,
+    hipLaunchKernelGGL((k_probe<R>), dim3(1), dim3(256), 0, s, (const R*)m.n2, (const lo_t*)m.nl2, 0,
                        c->plane, c->pz, c->probe[0], c->probe[1], c->probe[2], c->probe[3], c->probe[4],
                        c->probe[5], c->ring, c->cap, c->d_frame, m.step_base, m.step_off, c->every,
                        m.nsteps);
@@ -467,10 +467,10 @@ extern "C" int pf_get_field(pf_ctx* c, int32_t which, double* host, int64_t coun
     unsigned g = (unsigned)((n + 255) / 256);
     if (c->esz == 8)
         hipLaunchKernelGGL((k_pack<double>), dim3(g), dim3(256), 0, c->stream, (const double*)c->lvl[lvl][q],
-                           (const double*)nullptr, c->plane, c->pz, ex, ey, ez, (double*)c->scratch, subst, w);
+                           (const lo_t*)nullptr, c->plane, c->pz, ex, ey, ez, (double*)c->scratch, subst, w);
     else
         hipLaunchKernelGGL((k_pack<float>), dim3(g), dim3(256), 0, c->stream, (const float*)c->lvl[lvl][q],
-                           (const float*)c->lvlo[lvl][q], c->plane, c->pz, ex, ey, ez, (double*)c->scratch,
+                           (const lo_t*)c->lvlo[lvl][q], c->plane, c->pz, ex, ey, ez, (double*)c->scratch,
                            subst, w);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(host, c->scratch, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -494,10 +494,10 @@ extern "C" int pf_set_field(pf_ctx* c, int32_t which, const double* host, int64_
     unsigned g = (unsigned)((n + 255) / 256);
     if (c->esz == 8)
         hipLaunchKernelGGL((k_unpack<double>), dim3(g), dim3(256), 0, c->stream, (double*)c->lvl[lvl][q],
-                           (double*)nullptr, c->plane, c->pz, ex, ey, ez, (const double*)c->scratch);
+                           (lo_t*)nullptr, c->plane, c->pz, ex, ey, ez, (const double*)c->scratch);
     else
         hipLaunchKernelGGL((k_unpack<float>), dim3(g), dim3(256), 0, c->stream, (float*)c->lvl[lvl][q],
-                           (float*)c->lvlo[lvl][q], c->plane, c->pz, ex, ey, ez, (const double*)c->scratch);
+                           (lo_t*)c->lvlo[lvl][q], c->plane, c->pz, ex, ey, ez, (const double*)c->scratch);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     return PF_OK;
@@ -527,11 +527,12 @@ extern "C" int pf_get_probes(pf_ctx* c, double* host, int64_t cap_frames, int64_
 
 // ------------------------------------------------------------- energy ----
 template <class R>
-__global__ void k_widen(const R* __restrict__ hi, const R* __restrict__ lo, double* __restrict__ out, long long n) {
+__global__ void k_widen(const R* __restrict__ hi, const lo_t* __restrict__ lo, double* __restrict__ out, long long n) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    double v = (double)hi[t];
-    if (lo) v += (double)lo[t];
+    const R h = hi[t];
+    double v = (double)h;
+    if (lo) v += (double)lo_decode(h, lo[t]);
     out[t] = v;
 }
 
@@ -552,7 +553,7 @@ extern "C" int pf_energy(pf_ctx* c, double* out) {
             for (int q = 0; q < 3; q++) {
                 int lv = l == 0 ? newest : prev;
                 hipLaunchKernelGGL((k_widen<float>), dim3(g), dim3(256), 0, s, (const float*)c->lvl[lv][q],
-                                   (const float*)c->lvlo[lv][q], w + (size_t)(3 * l + q) * c->cells, c->cells);
+                                   (const lo_t*)c->lvlo[lv][q], w + (size_t)(3 * l + q) * c->cells, c->cells);
             }
         HIPCHK(hipGetLastError());
         Mesh<double>& m = ev.m;

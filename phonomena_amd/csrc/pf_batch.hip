@@ -465,7 +465,7 @@ extern "C" int pf_batch_add(pf_batch* b, pf_ctx* c) {
     if (c->precision != b->precision) return fail(PF_ERR_ARG, "context precision %d != batch precision %d", c->precision, b->precision);
     if (c->nmat > PF_TILED_MAX_MATERIALS)
         return fail(PF_ERR_ARG, "batched sweeps hold at most %d materials (got %d)", PF_TILED_MAX_MATERIALS, c->nmat);
-    if ((unsigned long long)c->cells * c->esz * (c->precision == PF_FP32C ? 6 : 3) >= (1ULL << 32))
+    if (level_bytes(c->cells, c->precision) >= (1ULL << 32))
         return fail(PF_ERR_ARG, "a batch member's time level must be < 4 GiB");
     const int sp = c->sponge ? 1 : 0;
     if (!b->members.empty() && b->sponge != sp)

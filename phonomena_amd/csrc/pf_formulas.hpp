@@ -113,6 +113,26 @@ PF_DEV R leap(R u, R uo, R coef, R dv, R s) {
 }
 
 // ---- compensated (hi + lo) arithmetic for PF_FP32C --------------------------
+// Storage of the lo part: after every normalising TwoSum |lo| <= ulp(hi)/2,
+// so lo is kept as a signed 16-bit multiple of 2^-16 ulp(hi) (ulp(hi) =
+// 2^(e-24) for hi = m 2^e, 0.5 <= |m| < 1): hi + lo carries 24 + 16
+// significant bits in 6 bytes instead of 8.  Both functions are exact
+// scalings (ldexp) around one round-to-nearest; hi = 0 has lo = 0.
+typedef int16_t lo_t;
+
+PF_DEV lo_t lo_encode(float hi, float lo) {
+    const int e = __builtin_amdgcn_frexp_expf(hi);
+    float q = __builtin_rintf(__builtin_amdgcn_ldexpf(lo, 40 - e));
+    q = fminf(fmaxf(q, -32767.0f), 32767.0f);
+    return (lo_t)(int)q;
+}
+PF_DEV float lo_decode(float hi, lo_t q) {
+    return __builtin_amdgcn_ldexpf((float)q, __builtin_amdgcn_frexp_expf(hi) - 40);
+}
+// FP64 fields carry no lo part.
+PF_DEV lo_t lo_encode(double, double) { return 0; }
+PF_DEV double lo_decode(double, lo_t) { return 0.0; }
+
 // Error-free transformations (Knuth TwoSum, FMA TwoProd); a value is the
 // unevaluated sum hi + lo with |lo| <= ulp(hi)/2 after normalisation.
 PF_DEV void two_sum(float a, float b, float& s, float& e) {

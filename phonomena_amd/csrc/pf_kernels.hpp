@@ -43,13 +43,13 @@ struct Mesh {
     R* n0;
     R* n1;
     R* n2;
-    // lo parts (PF_FP32C only)
-    const R* __restrict__ cl0;
-    const R* __restrict__ cl1;
-    const R* __restrict__ cl2;
-    R* nl0;
-    R* nl1;
-    R* nl2;
+    // lo parts (PF_FP32C only), 16-bit (lo_encode / lo_decode)
+    const lo_t* __restrict__ cl0;
+    const lo_t* __restrict__ cl1;
+    const lo_t* __restrict__ cl2;
+    lo_t* nl0;
+    lo_t* nl1;
+    lo_t* nl2;
     const uint8_t* __restrict__ ids;
     const R* __restrict__ mat;
     // spacing tables: FP64 = spacings, FP32 = inverse spacings
@@ -222,10 +222,10 @@ template <int Q, class R>
 PF_DEV V<R> cur_q(const Mesh<R>& m, int i, int j, int k, R w) {
     long long id = at(m, i, j, k);
     const R* c = Q == 0 ? m.c0 : (Q == 1 ? m.c1 : m.c2);
-    const R* cl = Q == 0 ? m.cl0 : (Q == 1 ? m.cl1 : m.cl2);
+    const lo_t* cl = Q == 0 ? m.cl0 : (Q == 1 ? m.cl1 : m.cl2);
     V<R> v;
     v.h = c[id];
-    v.l = cl ? cl[id] : R(0);
+    v.l = cl ? lo_decode(v.h, cl[id]) : R(0);
     if (Q == 2 && i == 0 && k == 0) { v.h = w; v.l = R(0); }
     return v;
 }
@@ -234,10 +234,10 @@ template <int Q, class R>
 PF_DEV V<R> new_q(const Mesh<R>& m, int i, int j, int k) {
     long long id = at(m, i, j, k);
     const R* n = Q == 0 ? m.n0 : (Q == 1 ? m.n1 : m.n2);
-    const R* nl = Q == 0 ? m.nl0 : (Q == 1 ? m.nl1 : m.nl2);
+    const lo_t* nl = Q == 0 ? m.nl0 : (Q == 1 ? m.nl1 : m.nl2);
     V<R> v;
     v.h = n[id];
-    v.l = nl ? nl[id] : R(0);
+    v.l = nl ? lo_decode(v.h, nl[id]) : R(0);
     return v;
 }
 
@@ -291,13 +291,13 @@ PF_DEV V<R> final_value(const Mesh<R>& m, int i, int j, int k, R w) {
 }
 
 template <int Q, class R> PF_DEV R* nptr(const Mesh<R>& m) { return Q == 0 ? m.n0 : (Q == 1 ? m.n1 : m.n2); }
-template <int Q, class R> PF_DEV R* nlptr(const Mesh<R>& m) { return Q == 0 ? m.nl0 : (Q == 1 ? m.nl1 : m.nl2); }
+template <int Q, class R> PF_DEV lo_t* nlptr(const Mesh<R>& m) { return Q == 0 ? m.nl0 : (Q == 1 ? m.nl1 : m.nl2); }
 
 template <int Q, int MODE, class R>
 PF_DEV void put(const Mesh<R>& m, int i, int j, int k, V<R> v) {
     const long long id = at(m, i, j, k);
     nptr<Q>(m)[id] = v.h;
-    if constexpr (MODE == MODE_FP32C) nlptr<Q>(m)[id] = v.l;
+    if constexpr (MODE == MODE_FP32C) nlptr<Q>(m)[id] = lo_encode(v.h, v.l);
 }
 
 // Write the stage-I value s0 of a non-face cell, and, when k = Lz-1, the
@@ -399,8 +399,9 @@ PF_DEV V<R> leap_from(V<R> cur, R oh, R ol, R coef, R dv, R s) {
 }
 
 template <int MODE, bool SPONGE, class R>
-PF_DEV V<R> leap_value(V<R> cur, const R* nh, const R* nl, long long id, R coef, R dv, R s) {
-    return leap_from<MODE, SPONGE>(cur, nh[id], nl ? nl[id] : R(0), coef, dv, s);
+PF_DEV V<R> leap_value(V<R> cur, const R* nh, const lo_t* nl, long long id, R coef, R dv, R s) {
+    const R oh = nh[id];
+    return leap_from<MODE, SPONGE>(cur, oh, nl ? lo_decode(oh, nl[id]) : R(0), coef, dv, s);
 }
 
 // ------------------------------------------------------- naive sweep ----
@@ -455,7 +456,7 @@ __global__ __launch_bounds__(256) void k_sweep_naive(Mesh<R> m) {
 // by all threads of one block; thread 0 advances the frame counter after
 // every read by the block barrier.
 template <class R>
-PF_DEV void probe_record(const R* __restrict__ uz, const R* __restrict__ uzl, long long plane, int pz,
+PF_DEV void probe_record(const R* __restrict__ uz, const lo_t* __restrict__ uzl, long long plane, int pz,
                          int i0, int si, int ni, int j0, int sj, int nj, double* __restrict__ ring, int cap,
                          long long* __restrict__ frame, long long tt, int every, long long nsteps) {
     if (tt >= nsteps || (tt + 1) % every != 0) return;
@@ -464,8 +465,9 @@ PF_DEV void probe_record(const R* __restrict__ uz, const R* __restrict__ uzl, lo
     for (int p = threadIdx.x; p < ni * nj; p += blockDim.x) {
         int a = p / nj, b = p % nj;
         long long id = (long long)(i0 + si * a) * plane + (long long)(j0 + sj * b) * pz;
-        double v = (double)uz[id];
-        if (uzl) v += (double)uzl[id];
+        const R h = uz[id];
+        double v = (double)h;
+        if (uzl) v += (double)lo_decode(h, uzl[id]);
         out[p] = v;
     }
     __syncthreads();
@@ -473,7 +475,7 @@ PF_DEV void probe_record(const R* __restrict__ uz, const R* __restrict__ uzl, lo
 }
 
 template <class R>
-__global__ __launch_bounds__(256) void k_probe(const R* __restrict__ uz, const R* __restrict__ uzl,
+__global__ __launch_bounds__(256) void k_probe(const R* __restrict__ uz, const lo_t* __restrict__ uzl,
                                                int plane_ny_pz_dummy, long long plane, int pz,
                                                int i0, int si, int ni, int j0, int sj, int nj,
                                                double* __restrict__ ring, int cap,
@@ -496,7 +498,7 @@ static __global__ void k_advance(long long* counter, long long by, long long cap
 // ------------------------------------------------------- pack/unpack ----
 // Dense reference-shaped double array <-> padded device component.
 template <class R>
-__global__ void k_pack(const R* __restrict__ hi, const R* __restrict__ lo, long long plane, int pz,
+__global__ void k_pack(const R* __restrict__ hi, const lo_t* __restrict__ lo, long long plane, int pz,
                        int ex, int ey, int ez, double* __restrict__ out,
                        int subst, double w) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -507,14 +509,15 @@ __global__ void k_pack(const R* __restrict__ hi, const R* __restrict__ lo, long 
     int j = (int)(r % ey);
     int i = (int)(r / ey);
     long long id = (long long)i * plane + (long long)j * pz + k;
-    double v = (double)hi[id];
-    if (lo) v += (double)lo[id];
+    const R h = hi[id];
+    double v = (double)h;
+    if (lo) v += (double)lo_decode(h, lo[id]);
     if (subst && i == 0 && k == 0) v = w;
     out[t] = v;
 }
 
 template <class R>
-__global__ void k_unpack(R* __restrict__ hi, R* __restrict__ lo, long long plane, int pz,
+__global__ void k_unpack(R* __restrict__ hi, lo_t* __restrict__ lo, long long plane, int pz,
                          int ex, int ey, int ez, const double* __restrict__ in) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long n = (long long)ex * ey * ez;
@@ -527,7 +530,7 @@ __global__ void k_unpack(R* __restrict__ hi, R* __restrict__ lo, long long plane
     double v = in[t];
     R h = (R)v;
     hi[id] = h;
-    if (lo) lo[id] = (R)(v - (double)h);
+    if (lo) lo[id] = lo_encode(h, (R)(v - (double)h));
 }
 
 }  // namespace pf

@@ -54,6 +54,14 @@ struct pf_ctx {
 
 namespace pf {
 
+// Bytes of one time level: three components of esz bytes, plus three
+// 16-bit lo parts in PF_FP32C.
+inline unsigned long long level_bytes(long long cells, int precision) {
+    const unsigned long long c = (unsigned long long)cells;
+    if (precision == PF_FP64) return c * 3 * 8;
+    return c * 3 * 4 + (precision == PF_FP32C ? c * 3 * sizeof(lo_t) : 0);
+}
+
 int fail(int code, const char* fmt, ...);
 int ctx_init(pf_ctx* c, const pf_desc* d);
 void ctx_free(pf_ctx* c);

@@ -131,12 +131,15 @@ __global__ __launch_bounds__(64 * W, (MODE == MODE_FP32 ? 4 : 2)) void k_sweep_s
                 o1[rr] = (m.n1 + pbase)[col[rr]];
                 o2[rr] = (m.n2 + pbase)[col[rr]];
                 if constexpr (COMP) {
-                    ol0[rr] = (m.nl0 + pbase)[col[rr]];
-                    ol1[rr] = (m.nl1 + pbase)[col[rr]];
-                    ol2[rr] = (m.nl2 + pbase)[col[rr]];
-                    cl0[rr] = (m.cl0 + pbase)[col[rr]];
-                    cl1[rr] = (m.cl1 + pbase)[col[rr]];
-                    cl2[rr] = (i == 0 && k == 0) ? R(0) : (m.cl2 + pbase)[col[rr]];
+                    ol0[rr] = lo_decode(o0[rr], (m.nl0 + pbase)[col[rr]]);
+                    ol1[rr] = lo_decode(o1[rr], (m.nl1 + pbase)[col[rr]]);
+                    ol2[rr] = lo_decode(o2[rr], (m.nl2 + pbase)[col[rr]]);
+                    // lo parts of the current level, decoded against the
+                    // stored hi of plane i
+                    cl0[rr] = lo_decode((m.c0 + pbase)[col[rr]], (m.cl0 + pbase)[col[rr]]);
+                    cl1[rr] = lo_decode((m.c1 + pbase)[col[rr]], (m.cl1 + pbase)[col[rr]]);
+                    cl2[rr] = (i == 0 && k == 0) ? R(0)
+                                                 : lo_decode((m.c2 + pbase)[col[rr]], (m.cl2 + pbase)[col[rr]]);
                 }
             }
         }

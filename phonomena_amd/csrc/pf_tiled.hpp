@@ -104,22 +104,32 @@ PF_DEV void bst(Rsrc r, unsigned off, unsigned soff, double v) {
     typedef unsigned int u2 __attribute__((ext_vector_type(2)));
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, (int)off, (int)soff, 0);
 }
+PF_DEV lo_t bld_lo(Rsrc r, unsigned off, unsigned soff) {
+    return (lo_t)__builtin_amdgcn_raw_buffer_load_b16(r, (int)off, (int)soff, 0);
+}
+PF_DEV void bst_lo(Rsrc r, unsigned off, unsigned soff, lo_t v) {
+    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v, r, (int)off, (int)soff, 0);
+}
 
 // A block-uniform value kept in a scalar register (plane offsets).
 PF_DEV unsigned uni(unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlane((int)v); }
 
-// Descriptors of the two levels and the index map; ab = bytes per array.
+// Descriptors of the two levels and the index map.  A level holds the three
+// hi arrays (ab bytes each) and, in FP32C, three 16-bit lo arrays (lb bytes
+// each) from byte 3 * ab on.
 struct Bufs {
     Rsrc cur, nxt, ids;
-    unsigned ab;
+    unsigned ab, lb;
 };
 template <class R, bool COMP> PF_DEV Bufs make_bufs(const Mesh<R>& m) {
     const long long n = (long long)m.nx * m.plane;
+    const long long lvl = n * 3 * (long long)sizeof(R) + (COMP ? n * 3 * (long long)sizeof(lo_t) : 0);
     Bufs b;
-    b.cur = arr_rsrc(m.c0, n * (COMP ? 6 : 3));
-    b.nxt = arr_rsrc(m.n0, n * (COMP ? 6 : 3));
+    b.cur = __builtin_amdgcn_make_buffer_rsrc(const_cast<R*>(m.c0), (short)0, (int)(unsigned)lvl, PF_RSRC_FLAGS);
+    b.nxt = __builtin_amdgcn_make_buffer_rsrc(m.n0, (short)0, (int)(unsigned)lvl, PF_RSRC_FLAGS);
     b.ids = arr_rsrc(m.ids, n);
     b.ab = uni((unsigned)(n * (long long)sizeof(R)));
+    b.lb = uni((unsigned)(n * (long long)sizeof(lo_t)));
     return b;
 }
 
@@ -165,28 +175,31 @@ PF_DEV Col<R> settle(const Raw<R>& r, bool valid, bool src, R w) {
 }
 
 // Values the displacement update of a plane reads at the node itself: the
-// previous level (hi, and lo in FP32C) and the current level's lo part.
+// previous level (hi, and the stored 16-bit lo in FP32C) and the current
+// level's stored lo part (decoded at use, against their hi).
 template <class R>
 struct Old {
-    R o0, o1, o2, ol0, ol1, ol2, cl0, cl1, cl2;
+    R o0, o1, o2;
+    lo_t ol0, ol1, ol2, cl0, cl1, cl2;
 };
 
 template <class R, bool COMP>
-PF_DEV Old<R> load_old(const Bufs& bf, int i, unsigned plane, unsigned boff) {
+PF_DEV Old<R> load_old(const Bufs& bf, int i, unsigned plane, unsigned boff, unsigned loff) {
     const unsigned soff = uni((unsigned)i * plane * (unsigned)sizeof(R));
     Old<R> o;
     o.o0 = bld(bf.nxt, boff, soff, R(0));
     o.o1 = bld(bf.nxt, boff, soff + bf.ab, R(0));
     o.o2 = bld(bf.nxt, boff, soff + 2 * bf.ab, R(0));
     if constexpr (COMP) {
-        o.ol0 = bld(bf.nxt, boff, soff + 3 * bf.ab, R(0));
-        o.ol1 = bld(bf.nxt, boff, soff + 4 * bf.ab, R(0));
-        o.ol2 = bld(bf.nxt, boff, soff + 5 * bf.ab, R(0));
-        o.cl0 = bld(bf.cur, boff, soff + 3 * bf.ab, R(0));
-        o.cl1 = bld(bf.cur, boff, soff + 4 * bf.ab, R(0));
-        o.cl2 = bld(bf.cur, boff, soff + 5 * bf.ab, R(0));
+        const unsigned sl = uni((unsigned)i * plane * (unsigned)sizeof(lo_t)) + 3 * bf.ab;
+        o.ol0 = bld_lo(bf.nxt, loff, sl);
+        o.ol1 = bld_lo(bf.nxt, loff, sl + bf.lb);
+        o.ol2 = bld_lo(bf.nxt, loff, sl + 2 * bf.lb);
+        o.cl0 = bld_lo(bf.cur, loff, sl);
+        o.cl1 = bld_lo(bf.cur, loff, sl + bf.lb);
+        o.cl2 = bld_lo(bf.cur, loff, sl + 2 * bf.lb);
     } else {
-        o.ol0 = o.ol1 = o.ol2 = o.cl0 = o.cl1 = o.cl2 = R(0);
+        o.ol0 = o.ol1 = o.ol2 = o.cl0 = o.cl1 = o.cl2 = 0;
     }
     return o;
 }
@@ -204,15 +217,15 @@ template <class R> PF_DEV R sel(bool c, R a, R b) { return c ? a : b; }
 // cell of the column (k = Lz - 1), the z = max Mur face cell above it:
 // face = mur(cur, s0, cur above), the cell above being the next lane.
 template <int Q, int MODE, class R>
-PF_DEV void put_tile(const Mesh<R>& m, const Bufs& bf, unsigned boff, unsigned soff, bool top, V<R> s0, V<R> cur,
-                     V<R> above) {
-    const unsigned sh = soff + Q * bf.ab, sl = soff + (3 + Q) * bf.ab;
+PF_DEV void put_tile(const Mesh<R>& m, const Bufs& bf, unsigned boff, unsigned loff, unsigned soff, unsigned slo,
+                     bool top, V<R> s0, V<R> cur, V<R> above) {
+    const unsigned sh = soff + Q * bf.ab, sl = slo + 3 * bf.ab + Q * bf.lb;
     bst(bf.nxt, boff, sh, s0.h);
-    if constexpr (MODE == MODE_FP32C) bst(bf.nxt, boff, sl, s0.l);
+    if constexpr (MODE == MODE_FP32C) bst_lo(bf.nxt, loff, sl, lo_encode(s0.h, s0.l));
     if (top) {
         const V<R> f = mur_map<MODE>(cur, m.mur[9 + Q], m.mur_lo[9 + Q], s0, above);
         bst(bf.nxt, boff + (unsigned)sizeof(R), sh, f.h);
-        if constexpr (MODE == MODE_FP32C) bst(bf.nxt, boff + (unsigned)sizeof(R), sl, f.l);
+        if constexpr (MODE == MODE_FP32C) bst_lo(bf.nxt, loff + (unsigned)sizeof(lo_t), sl, lo_encode(f.h, f.l));
     }
 }
 
@@ -297,6 +310,7 @@ PF_DEV void tiled_march(const Mesh<R>& m, TiledLds<R, TJ, KZ>& L, int kl, int ro
     }
     const unsigned col = valid ? (unsigned)(j * m.pz + k) : 0u;
     const unsigned boff = col * (unsigned)sizeof(R);
+    const unsigned loff = col * (unsigned)sizeof(lo_t);
     const unsigned plane = (unsigned)m.plane;
     // non-face cells per component (i-independent part); top = z-face below
     const bool wx = writer && jn && k <= m.nz - 2;
@@ -320,7 +334,7 @@ PF_DEV void tiled_march(const Mesh<R>& m, TiledLds<R, TJ, KZ>& L, int kl, int ro
     __builtin_amdgcn_sched_barrier(0);
     Raw<R> rb = load_raw<R>(bf, ib + 1 <= pmax ? ib + 1 : pmax, plane, boff, col);
     __builtin_amdgcn_sched_barrier(0);
-    Old<R> ob = load_old<R, COMP>(bf, ib, plane, boff);
+    Old<R> ob = load_old<R, COMP>(bf, ib, plane, boff, loff);
     __builtin_amdgcn_sched_barrier(0);
     Old<R> oa = {};  // the warm-up plane loads plane ib+1 into it
     L.su[1][0][row][kl] = c0.x;
@@ -400,37 +414,48 @@ PF_DEV void tiled_march(const Mesh<R>& m, TiledLds<R, TJ, KZ>& L, int kl, int ro
             const R s = SPONGE ? (xq0.dp + dpyz) + dpz_k : R(0);
             const R gx = sel(ks, xq0.hs, m.sx0);
             const unsigned soff = uni((unsigned)i * plane * (unsigned)sizeof(R));
-            const R cl2 = (!FULL && i == 0 && k == 0) ? R(0) : oin.cl2;
+            const unsigned slo = uni((unsigned)i * plane * (unsigned)sizeof(lo_t));
+            // lo parts decoded against their stored hi (c0 is the stored
+            // value except at the wall source line, whose lo is 0)
+            const R cl0 = COMP ? lo_decode(c0.x, oin.cl0) : R(0);
+            const R cl1 = COMP ? lo_decode(c0.y, oin.cl1) : R(0);
+            const R cl2 = (!COMP || (!FULL && i == 0 && k == 0)) ? R(0) : lo_decode(c0.z, oin.cl2);
+            const R ol0 = COMP ? lo_decode(oin.o0, oin.ol0) : R(0);
+            const R ol1 = COMP ? lo_decode(oin.o1, oin.ol1) : R(0);
+            const R ol2 = COMP ? lo_decode(oin.o2, oin.ol2) : R(0);
             const bool wall = !FULL && i == 0;  // uy, uz on the x = 0 wall stay 0
             // ux (base_solver.py:441-447, 495-501)
             {
                 const R dv = div3<EXACT>(n1_1 - n1_0, sel(ks, xq0.hf, m.fx0), s6 - s6_jm, hy_n,
                                          s5 - sel(ks, s5_km, R(0)), hz_n);
-                const V<R> cur = {c0.x, oin.cl0};
-                const V<R> s0 = leap_from<MODE, SPONGE>(cur, oin.o0, oin.ol0, mget(smat, c1.id, M_COEF), dv, s);
-                const V<R> above = {ux0_kp, COMP ? above0(oin.cl0) : R(0)};
-                if (wx && (FULL || i <= m.nx - 3)) put_tile<0, MODE>(m, bf, boff, soff, top_xy, s0, cur, above);
+                const V<R> cur = {c0.x, cl0};
+                const V<R> s0 = leap_from<MODE, SPONGE>(cur, oin.o0, ol0, mget(smat, c1.id, M_COEF), dv, s);
+                const V<R> above = {ux0_kp, COMP ? above0(cl0) : R(0)};
+                if (wx && (FULL || i <= m.nx - 3))
+                    put_tile<0, MODE>(m, bf, boff, loff, soff, slo, top_xy, s0, cur, above);
             }
             // uy (base_solver.py:449-455, 503-509)
             {
                 const R dv = div3<EXACT>(s6 - s6_m, gx, n2_0_jp - n2_0, uy_g2, s4 - sel(ks, s4_km, R(0)), hz_n);
-                const V<R> cur = {c0.y, oin.cl1};
-                V<R> s0 = leap_from<MODE, SPONGE>(cur, oin.o1, oin.ol1, mget(smat, id_jp, M_COEF), dv, s);
+                const V<R> cur = {c0.y, cl1};
+                V<R> s0 = leap_from<MODE, SPONGE>(cur, oin.o1, ol1, mget(smat, id_jp, M_COEF), dv, s);
                 if (wall) s0 = {R(0), R(0)};
-                const V<R> above = {uy0_kp, COMP ? above0(oin.cl1) : R(0)};
-                if (wy && (FULL || i <= m.nx - 2)) put_tile<1, MODE>(m, bf, boff, soff, top_xy, s0, cur, above);
+                const V<R> above = {uy0_kp, COMP ? above0(cl1) : R(0)};
+                if (wy && (FULL || i <= m.nx - 2))
+                    put_tile<1, MODE>(m, bf, boff, loff, soff, slo, top_xy, s0, cur, above);
             }
             // uz (base_solver.py:457-463, 511-517)
             {
                 const R dv = div3<EXACT>(s5 - s5_m, gx, s4 - s4_jm, hy_n, n3_0_kp - n3_0, uz_g3);
                 const V<R> cur = {c0.z, cl2};
-                V<R> s0 = leap_from<MODE, SPONGE>(cur, oin.o2, oin.ol2, mget(smat, ks ? id_kp : c0.id, M_COEF), dv, s);
+                V<R> s0 = leap_from<MODE, SPONGE>(cur, oin.o2, ol2, mget(smat, ks ? id_kp : c0.id, M_COEF), dv, s);
                 if (wall) s0 = {R(0), R(0)};
                 const V<R> above = {above0(c0.z), COMP ? above0(cl2) : R(0)};
-                if (wz && (FULL || i <= m.nx - 2)) put_tile<2, MODE>(m, bf, boff, soff, top_z, s0, cur, above);
+                if (wz && (FULL || i <= m.nx - 2))
+                    put_tile<2, MODE>(m, bf, boff, loff, soff, slo, top_z, s0, cur, above);
             }
         }
-        oin = load_old<R, COMP>(bf, i + 2 < ie ? i + 2 : ie - 1, plane, boff);
+        oin = load_old<R, COMP>(bf, i + 2 < ie ? i + 2 : ie - 1, plane, boff, loff);
         c0 = c1;
         n1_0 = n1_1;
         n2_0 = n2_1;
@@ -535,7 +560,7 @@ int launch_tiled_k(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
 
 template <class R, int MODE, bool SPONGE>
 int launch_tiled_s(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
-    if ((unsigned long long)c->cells * sizeof(R) * (MODE == MODE_FP32C ? 6 : 3) >= (1ULL << 32))
+    if (level_bytes(c->cells, c->precision) >= (1ULL << 32))
         return fail(PF_ERR_ARG, "tiled sweep: a time level must be < 4 GiB (use the naive kernel)");
     if (c->tile_x > XT - 3) return fail(PF_ERR_ARG, "tile_x must be <= %d (got %d)", XT - 3, c->tile_x);
     const int tj = c->tile_j, kz = c->tile_z;

@@ -27,8 +27,8 @@ import numpy as np
 from . import fdtd, sweep
 
 # bytes per node of a resident job: two time levels of three components
-# (hi, or hi + lo when compensated) and the material index
-NODE_BYTES = {"fp64": 2 * 3 * 8 + 1, "fp32": 2 * 3 * 4 + 1, "fp32c": 2 * 6 * 4 + 1}
+# (FP32 hi + 16-bit lo when compensated) and the material index
+NODE_BYTES = {"fp64": 2 * 3 * 8 + 1, "fp32": 2 * 3 * 4 + 1, "fp32c": 2 * 3 * (4 + 2) + 1}
 
 
 def resident_bytes(spec, precision):
