@@ -43,6 +43,10 @@ extern "C" {
 #define PF_KERNEL_TILED 2   /* LDS-staged (y,z) tiles, register march in x  */
 #define PF_KERNEL_STRIP 3   /* barrier-free: each wavefront owns a strip of
                                y rows in registers, z neighbours by DPP     */
+#define PF_KERNEL_TILED2 4  /* two-step tiled sweep (PF_FP32 / PF_FP32C):
+                               steps taken in pairs, levels n+1 and n+2 in
+                               one pass over memory; an odd step runs the
+                               one-step tiled sweep                         */
 
 /* Field selectors for pf_get_field / pf_set_field. */
 #define PF_UX 0
@@ -85,7 +89,8 @@ typedef struct pf_desc {
     int32_t record_every;      /* probe frame after every k-th step            */
     int32_t probe_capacity;    /* ring size in frames                          */
     int32_t tile_j;            /* rows per block (tiled: 4/8/16 with tile_z
-                                  64, 8/16/32 with tile_z 32) or per
+                                  64, 8/16/32 with tile_z 32; two-step: 16
+                                  with tile_z 64, 32 with tile_z 32) or per
                                   wavefront (strip: 4/8); 0 = default          */
     int32_t tile_x;            /* x planes per block (0 = automatic)           */
     int32_t tile_z;            /* tiled: lanes along z per tile row, 64 (one

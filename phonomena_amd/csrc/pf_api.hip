@@ -14,6 +14,7 @@
 #include "pf_kernels.hpp"
 #include "pf_runtime.hpp"
 #include "pf_strip.hpp"
+#include "pf_fused.hpp"
 #include "pf_tiled.hpp"
 
 using namespace pf;
@@ -81,7 +82,7 @@ int check_desc(const pf_desc* d) {
         return fail(PF_ERR_ARG, "node counts must be >= 4 per axis (got %d x %d x %d)", d->nx, d->ny, d->nz);
     if ((long long)d->nx * d->ny * d->nz > (1LL << 40)) return fail(PF_ERR_ARG, "mesh too large");
     if (d->precision < PF_FP64 || d->precision > PF_FP32C) return fail(PF_ERR_ARG, "bad precision %d", d->precision);
-    if (d->kernel < PF_KERNEL_AUTO || d->kernel > PF_KERNEL_STRIP) return fail(PF_ERR_ARG, "bad kernel %d", d->kernel);
+    if (d->kernel < PF_KERNEL_AUTO || d->kernel > PF_KERNEL_TILED2) return fail(PF_ERR_ARG, "bad kernel %d", d->kernel);
     if (d->n_materials < 1 || d->n_materials > 256) return fail(PF_ERR_ARG, "n_materials must be 1..256");
     if (!d->fdx || !d->fdy || !d->fdz || !d->sdx || !d->sdy || !d->sdz) return fail(PF_ERR_ARG, "spacing table missing");
     if (!d->materials || !d->ids) return fail(PF_ERR_ARG, "material table or ids missing");
@@ -163,10 +164,35 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
     // the tiled sweep addresses a whole level with 32-bit offsets
     const bool small = level_bytes(c->cells, d->precision) < (1ULL << 32);
     c->kernel = d->kernel == PF_KERNEL_AUTO ? (few_mats && small ? PF_KERNEL_TILED : PF_KERNEL_NAIVE) : d->kernel;
-    if ((c->kernel == PF_KERNEL_TILED || c->kernel == PF_KERNEL_STRIP) && !few_mats)
+    if (c->kernel != PF_KERNEL_NAIVE && !few_mats)
         return fail(PF_ERR_ARG, "the tiled and strip sweeps hold at most %d materials (got %d)",
                     PF_TILED_MAX_MATERIALS, d->n_materials);
     c->tile_x = d->tile_x > 0 ? d->tile_x : 0;
+    const bool pair = c->kernel == PF_KERNEL_TILED2;
+    if (pair) {
+        if (d->precision == PF_FP64)
+            return fail(PF_ERR_ARG, "the two-step sweep is built for PF_FP32 and PF_FP32C");
+        if (d->nx < 8 || d->ny < 8 || d->nz < 8)
+            return fail(PF_ERR_ARG, "the two-step sweep needs >= 8 nodes per axis (got %d x %d x %d)", d->nx, d->ny,
+                        d->nz);
+        if (!small) return fail(PF_ERR_ARG, "the two-step sweep needs a time level < 4 GiB");
+        if (c->tile_x > XT2 - 5) return fail(PF_ERR_ARG, "tile_x must be <= %d (got %d)", XT2 - 5, c->tile_x);
+        // 16 rows of 64 lanes or 32 rows of 32 lanes (1024 threads); by
+        // default the one with fewer idle lane-slots for this mesh
+        int kz = d->tile_z, tj = d->tile_j;
+        if (kz == 0 && tj != 0) kz = tj == 32 ? 32 : 64;
+        if (kz == 0) {
+            const TileMap a = fused_map(c, 16, 64, c->tile_x), b = fused_map(c, 32, 32, c->tile_x);
+            const double sa = (double)a.nkt * 64 * a.njt * 16 * a.nxc * (a.xchunk + 3);
+            const double sb = (double)b.nkt * 32 * b.njt * 32 * b.nxc * (b.xchunk + 3);
+            kz = sb <= sa ? 32 : 64;
+        }
+        if (tj == 0) tj = kz == 64 ? 16 : 32;
+        if (!((kz == 64 && tj == 16) || (kz == 32 && tj == 32)))
+            return fail(PF_ERR_ARG, "two-step tiles are 16 x 64 or 32 x 32 (got tile_j %d, tile_z %d)", tj, kz);
+        c->pair_j = tj;
+        c->pair_z = kz;
+    }
     if (c->kernel == PF_KERNEL_STRIP) {
         c->tile_j = d->tile_j > 0 ? d->tile_j : 4;
         if (c->tile_j != 4 && c->tile_j != 8)
@@ -178,18 +204,21 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
         // other layout is taken when it leaves 15% fewer lanes idle.
         const bool f64 = d->precision == PF_FP64;
         const int tj64 = f64 ? 8 : 16, tj32 = f64 ? 8 : 32;
-        int kz = d->tile_z;
+        // the one-step sweep of a two-step context (odd steps) takes its
+        // default shape
+        int kz = pair ? 0 : d->tile_z;
+        const int req_j = pair ? 0 : d->tile_j;
         if (kz != 0 && kz != 32 && kz != 64) return fail(PF_ERR_ARG, "tile_z must be 0, 32 or 64 (got %d)", kz);
         if (kz == 0) {
-            if (d->tile_j > 0) {
-                kz = d->tile_j == 32 ? 32 : 64;
+            if (req_j > 0) {
+                kz = req_j == 32 ? 32 : 64;
             } else {
                 const double s32 = pf::tiled_slots(c, tj32, 32), s64 = pf::tiled_slots(c, tj64, 64);
                 kz = f64 ? (s64 < 0.85 * s32 ? 64 : 32) : (s32 < 0.85 * s64 ? 32 : 64);
             }
         }
         c->tile_z = kz;
-        c->tile_j = d->tile_j > 0 ? d->tile_j : (kz == 64 ? tj64 : tj32);
+        c->tile_j = req_j > 0 ? req_j : (kz == 64 ? tj64 : tj32);
         const bool ok = kz == 64 ? (c->tile_j == 4 || c->tile_j == 8 || c->tile_j == 16)
                                  : (c->tile_j == 8 || c->tile_j == 16 || c->tile_j == 32);
         if (!ok) return fail(PF_ERR_ARG, "tile_j must be 4, 8 or 16 with tile_z 64, 8, 16 or 32 with tile_z 32 (got %d, %d)",
@@ -202,12 +231,13 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
     HIPCHK(hipGetDevice(&c->dev));
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
 
-    // fields: two levels x three components (+ 16-bit lo parts in
-    // compensated mode), each level one allocation
+    // fields: two levels (four for the two-step sweep) x three components
+    // (+ 16-bit lo parts in compensated mode), each level one allocation
     const size_t bytes = (size_t)c->cells * c->esz;
     const size_t lbytes = (size_t)c->cells * sizeof(lo_t);
     const size_t lvl_bytes = level_bytes(c->cells, c->precision);
-    for (int l = 0; l < 2; l++) {
+    c->nlevels = pair ? 4 : 2;
+    for (int l = 0; l < c->nlevels; l++) {
         HIPCHK(hipMalloc(&c->lvlbase[l], lvl_bytes));
         HIPCHK(hipMemset(c->lvlbase[l], 0, lvl_bytes));
         char* b = (char*)c->lvlbase[l];
@@ -216,6 +246,8 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
             if (c->precision == PF_FP32C) c->lvlo[l][q] = b + 3 * bytes + q * lbytes;
         }
     }
+    c->cur = 0;
+    c->old = 1;
     // material index map, in the field layout (rows unpadded: one copy; a
     // row-by-row 2-D copy took ~0.3 s per million nodes)
     HIPCHK(hipMalloc((void**)&c->ids, (size_t)c->cells));
@@ -281,12 +313,13 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
     c->e_blocks = 1024;
     HIPCHK(hipMalloc((void**)&c->e_partials, 2 * sizeof(double) * (size_t)c->e_blocks));
     HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipDeviceSynchronize());  // the blocking memsets above ran on the null stream
     return PF_OK;
 }
 
 void pf::ctx_free(pf_ctx* c) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
-    for (int l = 0; l < 2; l++)
+    for (int l = 0; l < 4; l++)
         F(c->lvlbase[l]);
     F(c->ids);
     F(c->mat); F(c->fx); F(c->fy); F(c->fz); F(c->sx); F(c->sy); F(c->sz);
@@ -308,14 +341,15 @@ extern "C" int pf_destroy(pf_ctx* c) {
 
 // ----------------------------------------------------------- stepping ----
 template <class R>
-Mesh<R> pf::make_view(const pf_ctx* c, int cur) {
+Mesh<R> pf::make_view(const pf_ctx* c, int cur, int old, int nxt) {
     Mesh<R> m;
     memset(&m, 0, sizeof m);
     m.nx = c->nx; m.ny = c->ny; m.nz = c->nz; m.pz = c->pz; m.plane = c->plane;
-    const int nxt = 1 - cur;
     m.c0 = (const R*)c->lvl[cur][0]; m.c1 = (const R*)c->lvl[cur][1]; m.c2 = (const R*)c->lvl[cur][2];
+    m.o0 = (const R*)c->lvl[old][0]; m.o1 = (const R*)c->lvl[old][1]; m.o2 = (const R*)c->lvl[old][2];
     m.n0 = (R*)c->lvl[nxt][0]; m.n1 = (R*)c->lvl[nxt][1]; m.n2 = (R*)c->lvl[nxt][2];
     m.cl0 = (const lo_t*)c->lvlo[cur][0]; m.cl1 = (const lo_t*)c->lvlo[cur][1]; m.cl2 = (const lo_t*)c->lvlo[cur][2];
+    m.ol0 = (const lo_t*)c->lvlo[old][0]; m.ol1 = (const lo_t*)c->lvlo[old][1]; m.ol2 = (const lo_t*)c->lvlo[old][2];
     m.nl0 = (lo_t*)c->lvlo[nxt][0]; m.nl1 = (lo_t*)c->lvlo[nxt][1]; m.nl2 = (lo_t*)c->lvlo[nxt][2];
     m.ids = c->ids;
     m.mat = (const R*)c->mat;
@@ -341,13 +375,13 @@ Mesh<R> pf::make_view(const pf_ctx* c, int cur) {
     return m;
 }
 
-template Mesh<float> pf::make_view<float>(const pf_ctx*, int);
-template Mesh<double> pf::make_view<double>(const pf_ctx*, int);
+template Mesh<float> pf::make_view<float>(const pf_ctx*, int, int, int);
+template Mesh<double> pf::make_view<double>(const pf_ctx*, int, int, int);
 
 // Launch the sweep of one step of mesh view m on stream s.
 template <class R, int MODE>
 static int launch_sweep_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
-    if (c->kernel == PF_KERNEL_TILED) return launch_tiled<R, MODE>(c, m, s);
+    if (c->kernel == PF_KERNEL_TILED || c->kernel == PF_KERNEL_TILED2) return launch_tiled<R, MODE>(c, m, s);
     if (c->kernel == PF_KERNEL_STRIP) return launch_strip<R, MODE>(c, m, s);
     dim3 blk(64, 4, 1);
     dim3 grd((unsigned)((c->nz + 63) / 64), (unsigned)((c->ny + 3) / 4), (unsigned)c->nx);
@@ -381,7 +415,7 @@ static int launch_probe_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
 template <class R, int MODE>
 static int step_once_t(pf_ctx* c, hipStream_t s, const long long* base, long long off,
                        hipEvent_t* ev /* 3 or null */) {
-    Mesh<R> m = make_view<R>(c, c->cur);
+    Mesh<R> m = make_view<R>(c, c->cur, c->old, c->old);
     m.step_base = base;
     m.step_off = off;
     int rc;
@@ -391,8 +425,69 @@ static int step_once_t(pf_ctx* c, hipStream_t s, const long long* base, long lon
     if ((rc = launch_faces_t<R, MODE>(c, m, s))) return rc;
     if (ev) HIPCHK(hipEventRecord(ev[2], s));
     if ((rc = launch_probe_t<R>(c, m, s))) return rc;
-    c->cur = 1 - c->cur;
+    const int t = c->cur;
+    c->cur = c->old;
+    c->old = t;
     return PF_OK;
+}
+
+template <class R, int MODE, bool SPONGE>
+static int launch_pair_s(const pf_ctx* c, const Mesh<R>& m, R* out2, hipStream_t s) {
+    const int tj = c->pair_j, kz = c->pair_z;
+    const TileMap tm = fused_map(c, tj, kz, c->tile_x);
+    const long long nb = (long long)tm.nkt * tm.njt * tm.nxc;
+    if (nb > 0x7fffffffLL) return fail(PF_ERR_ARG, "too many tiles");
+    if (kz == 64 && tj == 16)
+        hipLaunchKernelGGL((k_sweep_fused<R, MODE, SPONGE, 16, 64>), dim3((unsigned)nb), dim3(64, 16), 0, s, m, out2,
+                           tm);
+    else if (kz == 32 && tj == 32)
+        hipLaunchKernelGGL((k_sweep_fused<R, MODE, SPONGE, 32, 32>), dim3((unsigned)nb), dim3(64, 16), 0, s, m, out2,
+                           tm);
+    else
+        return fail(PF_ERR_ARG, "two-step sweep: tile_j %d with tile_z %d is not a built shape", tj, kz);
+    HIPCHK(hipGetLastError());
+    return PF_OK;
+}
+
+// Steps tt, tt+1: the two-step pass writes level n+1 (view m1: n, n-1 ->
+// a1) and the non-band cells of level n+2 (a2); then the level n+1 faces
+// and probe frame, the band of level n+2 (view m2: n+1, n -> a2, its
+// faces and probe frame.  a1, a2 are the two levels not in use.
+template <class R, int MODE>
+static int step_pair_t(pf_ctx* c, hipStream_t s, const long long* base, long long off) {
+    int a1 = -1, a2 = -1;
+    for (int l = 0; l < c->nlevels; l++)
+        if (l != c->cur && l != c->old) (a1 < 0 ? a1 : a2) = l;
+    if (a2 < 0) return fail(PF_ERR_STATE, "two-step sweep without its level buffers");
+    Mesh<R> m1 = make_view<R>(c, c->cur, c->old, a1);
+    m1.step_base = base;
+    m1.step_off = off;
+    Mesh<R> m2 = make_view<R>(c, a1, c->cur, a2);
+    m2.step_base = base;
+    m2.step_off = off + 1;
+    int rc = c->sponge ? launch_pair_s<R, MODE, true>(c, m1, m2.n0, s) : launch_pair_s<R, MODE, false>(c, m1, m2.n0, s);
+    if (rc) return rc;
+    if ((rc = launch_faces_t<R, MODE>(c, m1, s))) return rc;
+    if ((rc = launch_probe_t<R>(c, m1, s))) return rc;
+    const long long nb = band_total(c->nx, c->ny, c->nz);
+    if (c->sponge)
+        hipLaunchKernelGGL((k_band<R, MODE, true>), dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, m2, nb);
+    else
+        hipLaunchKernelGGL((k_band<R, MODE, false>), dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, m2, nb);
+    HIPCHK(hipGetLastError());
+    if ((rc = launch_faces_t<R, MODE>(c, m2, s))) return rc;
+    if ((rc = launch_probe_t<R>(c, m2, s))) return rc;
+    c->old = a1;
+    c->cur = a2;
+    return PF_OK;
+}
+
+int pf::step_pair(pf_ctx* c, hipStream_t s, const long long* base, long long off) {
+    switch (c->precision) {
+        case PF_FP32: return step_pair_t<float, MODE_FP32>(c, s, base, off);
+        case PF_FP32C: return step_pair_t<float, MODE_FP32C>(c, s, base, off);
+        default: return fail(PF_ERR_ARG, "the two-step sweep is built for PF_FP32 and PF_FP32C");
+    }
 }
 
 int pf::step_once(pf_ctx* c, hipStream_t s, const long long* base, long long off, hipEvent_t* ev) {
@@ -410,7 +505,15 @@ extern "C" int pf_step(pf_ctx* c, int64_t nsteps) {
     if (c->step + nsteps > c->wave_len)
         return fail(PF_ERR_RANGE, "step %lld + %lld beyond the source table (%lld)", (long long)c->step,
                     (long long)nsteps, (long long)c->wave_len);
-    for (int64_t s = 0; s < nsteps; s++) {
+    int64_t s = 0;
+    if (c->kernel == PF_KERNEL_TILED2) {
+        for (; s + 2 <= nsteps; s += 2) {
+            int rc = step_pair(c, c->stream, nullptr, c->step);
+            if (rc) return rc;
+            c->step += 2;
+        }
+    }
+    for (; s < nsteps; s++) {
         int rc = step_once(c, c->stream, nullptr, c->step, nullptr);
         if (rc) return rc;
         c->step++;
@@ -452,7 +555,7 @@ extern "C" int pf_get_field(pf_ctx* c, int32_t which, double* host, int64_t coun
     if (which < 0 || which > 5) return fail(PF_ERR_ARG, "bad field %d", which);
     if (c->batch) HIPCHK(hipStreamSynchronize(c->batch_stream));
     const int q = which % 3;
-    const int lvl = which < 3 ? c->cur : 1 - c->cur;
+    const int lvl = which < 3 ? c->cur : c->old;
     int ex, ey, ez;
     extents(c, q, ex, ey, ez);
     const long long n = (long long)ex * ey * ez;
@@ -483,7 +586,7 @@ extern "C" int pf_set_field(pf_ctx* c, int32_t which, const double* host, int64_
     if (which < 0 || which > 5) return fail(PF_ERR_ARG, "bad field %d", which);
     if (c->batch) return fail(PF_ERR_STATE, "context is stepped by its batch");
     const int q = which % 3;
-    const int lvl = which < 3 ? c->cur : 1 - c->cur;
+    const int lvl = which < 3 ? c->cur : c->old;
     int ex, ey, ez;
     extents(c, q, ex, ey, ez);
     const long long n = (long long)ex * ey * ez;
@@ -541,9 +644,9 @@ extern "C" int pf_energy(pf_ctx* c, double* out) {
     hipStream_t s = c->batch ? c->batch_stream : c->stream;
     EnergyView ev;
     memset(&ev, 0, sizeof ev);
-    const int newest = c->cur, prev = 1 - c->cur;
+    const int newest = c->cur, prev = c->old;
     if (c->esz == 8) {
-        ev.m = make_view<double>(c, newest);
+        ev.m = make_view<double>(c, newest, prev, prev);
     } else {
         int rc = ensure_scratch(c, (size_t)c->cells * sizeof(double) * 6);
         if (rc) return rc;

@@ -258,8 +258,10 @@ int build_t(pf_batch* b) {
         pf_ctx* c = b->members[(size_t)i];
         BMember<R>& e = mem[(size_t)i];
         memset(&e, 0, sizeof e);
+        // view[0]: the level current now; view[1]: after an odd count of steps
+        e.view[0] = make_view<R>(c, c->cur, c->old, c->old);
+        e.view[1] = make_view<R>(c, c->old, c->cur, c->cur);
         for (int p = 0; p < 2; p++) {
-            e.view[p] = make_view<R>(c, p);
             e.view[p].step_base = c->d_step;
             e.view[p].step_off = 0;
         }
@@ -269,7 +271,7 @@ int build_t(pf_batch* b) {
         e.tm = tile_map(c, tj, tz, xchunk);
         e.step = c->d_step;
         e.t0 = c->step;
-        e.cur0 = c->cur;
+        e.cur0 = 0;
         for (int q = 0; q < 6; q++) e.probe[q] = c->probe[q];
         e.every = c->every;
         e.cap = c->cap;
@@ -416,7 +418,11 @@ void account(pf_batch* b, long long n) {
         const long long left = c->wave_len - c->step;
         const long long take = n < left ? n : (left > 0 ? left : 0);
         c->step += take;
-        c->cur ^= (int)(take & 1);
+        if (take & 1) {
+            const int t = c->cur;
+            c->cur = c->old;
+            c->old = t;
+        }
     }
 }
 

@@ -39,10 +39,15 @@ struct Mesh {
     const R* __restrict__ c0;
     const R* __restrict__ c1;
     const R* __restrict__ c2;
-    // level n-1 on entry, level n+1 on exit, hi part
+    // level written by the step (level n+1), hi part
     R* n0;
     R* n1;
     R* n2;
+    // level n-1 (read at the node only; the same arrays as n* when the step
+    // overwrites the previous level in place)
+    const R* o0;
+    const R* o1;
+    const R* o2;
     // lo parts (PF_FP32C only), 16-bit (lo_encode / lo_decode)
     const lo_t* __restrict__ cl0;
     const lo_t* __restrict__ cl1;
@@ -50,6 +55,9 @@ struct Mesh {
     lo_t* nl0;
     lo_t* nl1;
     lo_t* nl2;
+    const lo_t* ol0;
+    const lo_t* ol1;
+    const lo_t* ol2;
     const uint8_t* __restrict__ ids;
     const R* __restrict__ mat;
     // spacing tables: FP64 = spacings, FP32 = inverse spacings
@@ -405,18 +413,12 @@ PF_DEV V<R> leap_value(V<R> cur, const R* nh, const lo_t* nl, long long id, R co
 }
 
 // ------------------------------------------------------- naive sweep ----
-// One thread per node; every stress a stencil needs is recomputed from u.
-// Writes the non-face cells and the z = max face cells above them.
+// One node (i, j, k) of a step: every stress a stencil needs is recomputed
+// from u.  Writes the node's non-face cells and the z = max face cells
+// above them.
 template <class R, int MODE, bool SPONGE>
-__global__ __launch_bounds__(256) void k_sweep_naive(Mesh<R> m) {
+PF_DEV void naive_cell(const Mesh<R>& m, int i, int j, int k, R w) {
     constexpr bool EXACT = MODE == MODE_FP64;
-    const int k = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    const int i = blockIdx.z;
-    if (k >= m.nz || j >= m.ny || i >= m.nx) return;
-    const long long tt = step_index(m);
-    if (tt >= m.nsteps) return;
-    const R w = (R)m.wave[tt];
     const long long id = at(m, i, j, k);
     const R s = SPONGE ? damp_at(m, i, j, k) : R(0);
     // ux: non-face cells i <= nx-3, 1 <= j <= ny-2, k <= nz-2
@@ -424,7 +426,7 @@ __global__ __launch_bounds__(256) void k_sweep_naive(Mesh<R> m) {
         R coef, dv;
         stage_ux<EXACT>(m, i, j, k, w, coef, dv);
         V<R> cur = cur_q<0>(m, i, j, k, w);
-        put_with_zface<0, MODE>(m, i, j, k, leap_value<MODE, SPONGE>(cur, m.n0, m.nl0, id, coef, dv, s), cur, w);
+        put_with_zface<0, MODE>(m, i, j, k, leap_value<MODE, SPONGE>(cur, m.o0, m.ol0, id, coef, dv, s), cur, w);
     }
     // uy: non-face cells i <= nx-2, 1 <= j <= ny-3, k <= nz-2; wall i = 0 -> 0
     if (i <= m.nx - 2 && j >= 1 && j <= m.ny - 3 && k <= m.nz - 2) {
@@ -433,7 +435,7 @@ __global__ __launch_bounds__(256) void k_sweep_naive(Mesh<R> m) {
         if (i > 0) {
             R coef, dv;
             stage_uy<EXACT>(m, i, j, k, w, coef, dv);
-            s0 = leap_value<MODE, SPONGE>(cur, m.n1, m.nl1, id, coef, dv, s);
+            s0 = leap_value<MODE, SPONGE>(cur, m.o1, m.ol1, id, coef, dv, s);
         }
         put_with_zface<1, MODE>(m, i, j, k, s0, cur, w);
     }
@@ -444,10 +446,22 @@ __global__ __launch_bounds__(256) void k_sweep_naive(Mesh<R> m) {
         if (i > 0) {
             R coef, dv;
             stage_uz<EXACT>(m, i, j, k, w, coef, dv);
-            s0 = leap_value<MODE, SPONGE>(cur, m.n2, m.nl2, id, coef, dv, s);
+            s0 = leap_value<MODE, SPONGE>(cur, m.o2, m.ol2, id, coef, dv, s);
         }
         put_with_zface<2, MODE>(m, i, j, k, s0, cur, w);
     }
+}
+
+// One thread per node.
+template <class R, int MODE, bool SPONGE>
+__global__ __launch_bounds__(256) void k_sweep_naive(Mesh<R> m) {
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k >= m.nz || j >= m.ny || i >= m.nx) return;
+    const long long tt = step_index(m);
+    if (tt >= m.nsteps) return;
+    naive_cell<R, MODE, SPONGE>(m, i, j, k, (R)m.wave[tt]);
 }
 
 // ------------------------------------------------------------ probes ----

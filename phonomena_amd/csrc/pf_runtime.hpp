@@ -8,24 +8,29 @@
 // Material rows the tiled sweep stages in LDS.
 #define PF_TILED_MAX_MATERIALS 16
 
-// One mesh on one device.  Two levels of each displacement component are
-// kept (ping-pong); `cur` indexes the current level.  Each level is one
-// allocation holding its arrays back to back (ux, uy, uz, then the lo parts
-// in FP32C), so one buffer descriptor covers a whole level.
+// One mesh on one device.  Time levels of the displacement components are
+// kept in level buffers: `cur` indexes the current level (n) and `old` the
+// previous one (n-1); a single step overwrites `old` with level n+1 and
+// swaps the two.  Two-step sweeps use two more level buffers (allocated on
+// first use).  Each level is one allocation holding its arrays back to back
+// (ux, uy, uz, then the 16-bit lo parts in FP32C), so one buffer descriptor
+// covers a whole level.
 struct pf_ctx {
     int nx = 0, ny = 0, nz = 0, pz = 0;
     long long plane = 0, cells = 0;  // ny*pz, nx*ny*pz
     int precision = 0, kernel = 0, tile_j = 4, tile_x = 0, tile_z = 64;
+    int pair_j = 0, pair_z = 0;  // tile of the two-step sweep (PF_KERNEL_TILED2)
     size_t esz = 8;
     bool sponge = false;
     double dt = 0;
     int nmat = 0;
     int dev = 0;
     hipStream_t stream = nullptr;
-    void* lvlbase[2] = {};
-    void* lvl[2][3] = {};
-    void* lvlo[2][3] = {};
-    int cur = 0;
+    void* lvlbase[4] = {};
+    void* lvl[4][3] = {};
+    void* lvlo[4][3] = {};
+    int nlevels = 0;
+    int cur = 0, old = 1;
     uint8_t* ids = nullptr;
     void *mat = nullptr, *fx = nullptr, *fy = nullptr, *fz = nullptr, *sx = nullptr, *sy = nullptr,
          *sz = nullptr;
@@ -65,11 +70,14 @@ inline unsigned long long level_bytes(long long cells, int precision) {
 int fail(int code, const char* fmt, ...);
 int ctx_init(pf_ctx* c, const pf_desc* d);
 void ctx_free(pf_ctx* c);
+// View of a step from level `cur` (n) and `old` (n-1) into level `out`.
 template <class R>
-Mesh<R> make_view(const pf_ctx* c, int cur);
+Mesh<R> make_view(const pf_ctx* c, int cur, int old, int out);
 // Issue one step (sweep, faces, probe) on stream s; tt = *base + off (or
 // off when base is null).  ev: 3 events recorded around the launches, or null.
 int step_once(pf_ctx* c, hipStream_t s, const long long* base, long long off, hipEvent_t* ev);
+// Issue steps tt and tt+1 with the two-step sweep (PF_KERNEL_TILED2).
+int step_pair(pf_ctx* c, hipStream_t s, const long long* base, long long off);
 
 }  // namespace pf
 

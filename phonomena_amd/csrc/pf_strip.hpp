@@ -127,13 +127,13 @@ __global__ __launch_bounds__(64 * W, (MODE == MODE_FP32 ? 4 : 2)) void k_sweep_s
         for (int rr = 1; rr <= NR; ++rr) {
             o0[rr] = o1[rr] = o2[rr] = ol0[rr] = ol1[rr] = ol2[rr] = cl0[rr] = cl1[rr] = cl2[rr] = R(0);
             if (outp && rv[rr] && klane) {
-                o0[rr] = (m.n0 + pbase)[col[rr]];
-                o1[rr] = (m.n1 + pbase)[col[rr]];
-                o2[rr] = (m.n2 + pbase)[col[rr]];
+                o0[rr] = (m.o0 + pbase)[col[rr]];
+                o1[rr] = (m.o1 + pbase)[col[rr]];
+                o2[rr] = (m.o2 + pbase)[col[rr]];
                 if constexpr (COMP) {
-                    ol0[rr] = lo_decode(o0[rr], (m.nl0 + pbase)[col[rr]]);
-                    ol1[rr] = lo_decode(o1[rr], (m.nl1 + pbase)[col[rr]]);
-                    ol2[rr] = lo_decode(o2[rr], (m.nl2 + pbase)[col[rr]]);
+                    ol0[rr] = lo_decode(o0[rr], (m.ol0 + pbase)[col[rr]]);
+                    ol1[rr] = lo_decode(o1[rr], (m.ol1 + pbase)[col[rr]]);
+                    ol2[rr] = lo_decode(o2[rr], (m.ol2 + pbase)[col[rr]]);
                     // lo parts of the current level, decoded against the
                     // stored hi of plane i
                     cl0[rr] = lo_decode((m.c0 + pbase)[col[rr]], (m.cl0 + pbase)[col[rr]]);

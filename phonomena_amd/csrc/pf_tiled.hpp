@@ -118,7 +118,7 @@ PF_DEV unsigned uni(unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlan
 // hi arrays (ab bytes each) and, in FP32C, three 16-bit lo arrays (lb bytes
 // each) from byte 3 * ab on.
 struct Bufs {
-    Rsrc cur, nxt, ids;
+    Rsrc cur, old, nxt, ids;  // level n (read), n-1 (read), n+1 (written)
     unsigned ab, lb;
 };
 template <class R, bool COMP> PF_DEV Bufs make_bufs(const Mesh<R>& m) {
@@ -126,6 +126,7 @@ template <class R, bool COMP> PF_DEV Bufs make_bufs(const Mesh<R>& m) {
     const long long lvl = n * 3 * (long long)sizeof(R) + (COMP ? n * 3 * (long long)sizeof(lo_t) : 0);
     Bufs b;
     b.cur = __builtin_amdgcn_make_buffer_rsrc(const_cast<R*>(m.c0), (short)0, (int)(unsigned)lvl, PF_RSRC_FLAGS);
+    b.old = __builtin_amdgcn_make_buffer_rsrc(const_cast<R*>(m.o0), (short)0, (int)(unsigned)lvl, PF_RSRC_FLAGS);
     b.nxt = __builtin_amdgcn_make_buffer_rsrc(m.n0, (short)0, (int)(unsigned)lvl, PF_RSRC_FLAGS);
     b.ids = arr_rsrc(m.ids, n);
     b.ab = uni((unsigned)(n * (long long)sizeof(R)));
@@ -187,14 +188,14 @@ template <class R, bool COMP>
 PF_DEV Old<R> load_old(const Bufs& bf, int i, unsigned plane, unsigned boff, unsigned loff) {
     const unsigned soff = uni((unsigned)i * plane * (unsigned)sizeof(R));
     Old<R> o;
-    o.o0 = bld(bf.nxt, boff, soff, R(0));
-    o.o1 = bld(bf.nxt, boff, soff + bf.ab, R(0));
-    o.o2 = bld(bf.nxt, boff, soff + 2 * bf.ab, R(0));
+    o.o0 = bld(bf.old, boff, soff, R(0));
+    o.o1 = bld(bf.old, boff, soff + bf.ab, R(0));
+    o.o2 = bld(bf.old, boff, soff + 2 * bf.ab, R(0));
     if constexpr (COMP) {
         const unsigned sl = uni((unsigned)i * plane * (unsigned)sizeof(lo_t)) + 3 * bf.ab;
-        o.ol0 = bld_lo(bf.nxt, loff, sl);
-        o.ol1 = bld_lo(bf.nxt, loff, sl + bf.lb);
-        o.ol2 = bld_lo(bf.nxt, loff, sl + 2 * bf.lb);
+        o.ol0 = bld_lo(bf.old, loff, sl);
+        o.ol1 = bld_lo(bf.old, loff, sl + bf.lb);
+        o.ol2 = bld_lo(bf.old, loff, sl + 2 * bf.lb);
         o.cl0 = bld_lo(bf.cur, loff, sl);
         o.cl1 = bld_lo(bf.cur, loff, sl + bf.lb);
         o.cl2 = bld_lo(bf.cur, loff, sl + 2 * bf.lb);

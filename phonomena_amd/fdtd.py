@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libphonofdtd.so")
 PF_OK = 0
 ERRORS = {-1: "PF_ERR_ARG", -2: "PF_ERR_HIP", -3: "PF_ERR_STATE", -4: "PF_ERR_NOMEM", -5: "PF_ERR_RANGE"}
 PRECISION = {"fp64": 0, "fp32": 1, "fp32c": 2}
-KERNEL = {"auto": 0, "naive": 1, "tiled": 2, "strip": 3}
+KERNEL = {"auto": 0, "naive": 1, "tiled": 2, "strip": 3, "tiled2": 4}
 FIELDS = {"ux": 0, "uy": 1, "uz": 2, "ux_old": 3, "uy_old": 4, "uz_old": 5}
 
 
