@@ -269,7 +269,7 @@ def main():
     ap.add_argument("--workload", default="plain", choices=sorted(WORKLOADS))
     ap.add_argument("--no-sponge", action="store_true", help="Mur faces only, no absorbing layer")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32c", "fp64"])
-    ap.add_argument("--kernel", default="auto", choices=["auto", "naive", "tiled", "strip"])
+    ap.add_argument("--kernel", default="auto", choices=["auto", "naive", "tiled", "strip", "tiled2"])
     ap.add_argument("--tile-j", type=int, default=0, help="tiled sweep rows per block (4..32)")
     ap.add_argument("--tile-z", type=int, default=0, help="tiled sweep lanes along z per row (32, 64)")
     ap.add_argument("--tile-x", type=int, default=0, help="tiled sweep x planes per block")

@@ -129,7 +129,9 @@ int pf_energy(pf_ctx *ctx, double *out);
 
 /*
  * Timing: run nsteps steps with HIP events around every sweep and face
- * launch on the context's stream; returns the mean duration of each (ms).
+ * launch on the context's stream; returns the mean duration of each per
+ * step (ms).  With PF_KERNEL_TILED2, sweep = the two-step pass and face =
+ * the faces, probes and band of both steps, each divided by 2 per pair.
  */
 int pf_profile(pf_ctx *ctx, int64_t nsteps, double *sweep_ms, double *face_ms);
 

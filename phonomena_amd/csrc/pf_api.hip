@@ -176,7 +176,7 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
             return fail(PF_ERR_ARG, "the two-step sweep needs >= 8 nodes per axis (got %d x %d x %d)", d->nx, d->ny,
                         d->nz);
         if (!small) return fail(PF_ERR_ARG, "the two-step sweep needs a time level < 4 GiB");
-        if (c->tile_x > XT2 - 5) return fail(PF_ERR_ARG, "tile_x must be <= %d (got %d)", XT2 - 5, c->tile_x);
+        if (c->tile_x > XT2 - 6) return fail(PF_ERR_ARG, "tile_x must be <= %d (got %d)", XT2 - 6, c->tile_x);
         // 16 rows of 64 lanes or 32 rows of 32 lanes (1024 threads); by
         // default the one with fewer idle lane-slots for this mesh
         int kz = d->tile_z, tj = d->tile_j;
@@ -454,7 +454,7 @@ static int launch_pair_s(const pf_ctx* c, const Mesh<R>& m, R* out2, hipStream_t
 // and probe frame, the band of level n+2 (view m2: n+1, n -> a2, its
 // faces and probe frame.  a1, a2 are the two levels not in use.
 template <class R, int MODE>
-static int step_pair_t(pf_ctx* c, hipStream_t s, const long long* base, long long off) {
+static int step_pair_t(pf_ctx* c, hipStream_t s, const long long* base, long long off, hipEvent_t* ev) {
     int a1 = -1, a2 = -1;
     for (int l = 0; l < c->nlevels; l++)
         if (l != c->cur && l != c->old) (a1 < 0 ? a1 : a2) = l;
@@ -465,8 +465,10 @@ static int step_pair_t(pf_ctx* c, hipStream_t s, const long long* base, long lon
     Mesh<R> m2 = make_view<R>(c, a1, c->cur, a2);
     m2.step_base = base;
     m2.step_off = off + 1;
+    if (ev) HIPCHK(hipEventRecord(ev[0], s));
     int rc = c->sponge ? launch_pair_s<R, MODE, true>(c, m1, m2.n0, s) : launch_pair_s<R, MODE, false>(c, m1, m2.n0, s);
     if (rc) return rc;
+    if (ev) HIPCHK(hipEventRecord(ev[1], s));
     if ((rc = launch_faces_t<R, MODE>(c, m1, s))) return rc;
     if ((rc = launch_probe_t<R>(c, m1, s))) return rc;
     const long long nb = band_total(c->nx, c->ny, c->nz);
@@ -477,15 +479,16 @@ static int step_pair_t(pf_ctx* c, hipStream_t s, const long long* base, long lon
     HIPCHK(hipGetLastError());
     if ((rc = launch_faces_t<R, MODE>(c, m2, s))) return rc;
     if ((rc = launch_probe_t<R>(c, m2, s))) return rc;
+    if (ev) HIPCHK(hipEventRecord(ev[2], s));
     c->old = a1;
     c->cur = a2;
     return PF_OK;
 }
 
-int pf::step_pair(pf_ctx* c, hipStream_t s, const long long* base, long long off) {
+int pf::step_pair(pf_ctx* c, hipStream_t s, const long long* base, long long off, hipEvent_t* ev) {
     switch (c->precision) {
-        case PF_FP32: return step_pair_t<float, MODE_FP32>(c, s, base, off);
-        case PF_FP32C: return step_pair_t<float, MODE_FP32C>(c, s, base, off);
+        case PF_FP32: return step_pair_t<float, MODE_FP32>(c, s, base, off, ev);
+        case PF_FP32C: return step_pair_t<float, MODE_FP32C>(c, s, base, off, ev);
         default: return fail(PF_ERR_ARG, "the two-step sweep is built for PF_FP32 and PF_FP32C");
     }
 }
@@ -508,7 +511,7 @@ extern "C" int pf_step(pf_ctx* c, int64_t nsteps) {
     int64_t s = 0;
     if (c->kernel == PF_KERNEL_TILED2) {
         for (; s + 2 <= nsteps; s += 2) {
-            int rc = step_pair(c, c->stream, nullptr, c->step);
+            int rc = step_pair(c, c->stream, nullptr, c->step, nullptr);
             if (rc) return rc;
             c->step += 2;
         }
@@ -693,17 +696,27 @@ extern "C" int pf_profile(pf_ctx* c, int64_t nsteps, double* sweep_ms, double* f
     if (c->batch) return fail(PF_ERR_STATE, "context is stepped by its batch");
     if (nsteps < 1) return fail(PF_ERR_ARG, "nsteps < 1");
     if (c->step + nsteps > c->wave_len) return fail(PF_ERR_RANGE, "beyond the source table");
+    // three events per launch group: a single step, or a pair of steps of
+    // the two-step sweep (sweep time = the two-step pass, the rest = faces,
+    // probes and band of both steps)
     std::vector<hipEvent_t> ev((size_t)nsteps * 3);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     int rc = PF_OK;
-    for (int64_t s = 0; s < nsteps && !rc; s++) {
-        rc = step_once(c, c->stream, nullptr, c->step, &ev[(size_t)s * 3]);
+    int64_t groups = 0, s = 0;
+    if (c->kernel == PF_KERNEL_TILED2) {
+        for (; s + 2 <= nsteps && !rc; s += 2, groups++) {
+            rc = step_pair(c, c->stream, nullptr, c->step, &ev[(size_t)groups * 3]);
+            if (!rc) c->step += 2;
+        }
+    }
+    for (; s < nsteps && !rc; s++, groups++) {
+        rc = step_once(c, c->stream, nullptr, c->step, &ev[(size_t)groups * 3]);
         if (!rc) c->step++;
     }
     if (!rc) {
         HIPCHK(hipStreamSynchronize(c->stream));
         double sw = 0, fc = 0;
-        for (int64_t s = 0; s < nsteps; s++) {
+        for (int64_t s = 0; s < groups; s++) {
             float a = 0, b = 0;
             HIPCHK(hipEventElapsedTime(&a, ev[(size_t)s * 3], ev[(size_t)s * 3 + 1]));
             HIPCHK(hipEventElapsedTime(&b, ev[(size_t)s * 3 + 1], ev[(size_t)s * 3 + 2]));

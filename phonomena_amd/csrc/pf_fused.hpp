@@ -3,8 +3,8 @@
 // The block marches through its chunk of x planes as the one-step tiled
 // sweep does (stage A: level n+1 of plane i from levels n and n-1), and a
 // second stage B follows one plane behind: level n+2 of plane i-1 from the
-// level n+1 values stage A has just computed (taken from registers and LDS,
-// never re-read from memory) and level n (the node's own value, still in
+// level n+1 values stage A has computed (kept in registers and LDS, never
+// re-read from memory) and level n (the node's own value, still in
 // registers from stage A).  Per two steps the pass reads levels n and n-1
 // and the material indices once and writes levels n+1 and n+2 once: 49 B
 // per node in FP32 where two one-step sweeps move 2 x 37.
@@ -17,18 +17,19 @@
 // are only read and levels n+1 and n+2 go to two other level buffers, so
 // no block reads what another one writes.
 //
-// Boundary faces.  Level n+1 on the Mur faces is only final after the face
-// kernel has run, so stage B stores only cells whose stencil reads no face
-// cell of level n+1: i <= nx-4, 2 <= j <= ny-4, k <= nz-4 (a face cell of
-// some component lies at i >= nx-2, j = 0, j >= ny-2 or k >= nz-2).  The
-// one-step per-cell code updates the remaining band after the level n+1
-// faces (k_band), then the level n+2 faces are written.
+// Boundary faces.  The z = max face cells of level n+1 are a single Mur map
+// of the cell below (which stage A writes with it), so stage B takes them
+// from the lane below and covers the top of the columns.  The x and y faces
+// are final only after the face kernel, so stage B stores no cell whose
+// stencil reads one of them: it stores i <= nx-4, 2 <= j <= ny-4.  The
+// one-step per-cell code updates the remaining band (k_band) after the
+// level n+1 faces, then the level n+2 faces are written.
 //
 // Stage B sees level n+1 exactly as a one-step sweep loads it: 0 outside
-// the mesh, the source value w(tt+1) on the wall line uz(0, j, 0), lo parts
-// rounded to their 16-bit storage.  Every formula is the shared per-value
-// code of the one-step sweep fed the same operands, so a pass gives the
-// bits of two one-step sweeps.
+// the mesh and on never-written cells, the source value w(tt+1) on the wall
+// line uz(0, j, 0), lo parts rounded to their 16-bit storage.  Every
+// formula is the shared per-value code of the one-step sweep fed the same
+// operands, so a pass gives the bits of two one-step sweeps.
 //
 // Barriers: three per plane.  Per plane i (H = LDS parity of plane i+1):
 //   A1  N at plane i+1, T at plane i from level n       -> sn2[H], ss4, ss6
@@ -41,13 +42,15 @@
 //   B2  level n+2 of plane i-1; stores
 // Every LDS array is written in one interval and read in the one or two
 // after it, before its next write (sid has four slots, plane & 3, so the
-// write of plane i+2 in B1 does not touch planes i-1 .. i+1).
+// write of plane i+2 in B1 does not touch planes i-1 .. i+1).  A version
+// with stage B two planes behind needs only two barriers per plane but
+// more registers than a 1024-thread block has (it spilled and ran slower).
 #pragma once
 #include "pf_tiled.hpp"
 
 namespace pf {
 
-constexpr int XT2 = 256;  // x-table entries of the two-step sweep: planes per chunk + 5
+constexpr int XT2 = 256;  // x-table entries of the two-step sweep: planes per chunk + 6
 
 template <class R, int TJ, int KZ>
 struct FusedLds {
@@ -61,7 +64,7 @@ struct FusedLds {
     R sbs6[TJ][KZ];
     uint16_t sid[4][TJ][KZ];  // material row offsets, by plane & 3
     R smat[PF_TILED_MAX_MATERIALS * MAT_STRIDE];
-    XPlane<R> sxp[XT2];    // planes ib-2 .. ie+2
+    XPlane<R> sxp[XT2];    // planes ib-2 .. ie+3
 };
 
 // Level n+1 of a node as a one-step sweep loads it: hi as stored, lo
@@ -133,13 +136,17 @@ PF_DEV void fused_march(const Mesh<R>& m, R* out2, FusedLds<R, TJ, KZ>& L, int k
     const bool wz = own && jn && k <= m.nz - 3;
     const bool top_xy = !FULL && k == m.nz - 2;
     const bool top_z = !FULL && k == m.nz - 3;
-    // level n+2 stores: owned cells off the band
-    const bool wb = own && (FULL || (j >= 2 && j <= m.ny - 4 && k <= m.nz - 4));
+    // level n+2 is stored off the x and y bands
+    const bool wb = FULL || (j >= 2 && j <= m.ny - 4);
 
     const Bufs bf = make_bufs<R, COMP>(m);  // cur = n, old = n-1, nxt = n+1
     Bufs bf2 = bf;
     bf2.nxt = level_rsrc<R, COMP>(m, out2);  // n+2
 
+    // stage A's values are used on all but the outer ring of the tile
+    const bool useo = valid && kl >= 1 && kl <= KZ - 2 && row >= 1 && row <= TJ - 2;
+    const unsigned ooff = useo || !bf.far ? boff : bf.far;
+    const unsigned oloff = useo || !bf.far ? loff : bf.far;
     const int pmax = ie + 1 < m.nx ? ie + 1 : m.nx - 1;  // last level-n plane read
     const int omax = ie < m.nx ? ie : m.nx - 1;          // last plane whose node values are read
 
@@ -155,7 +162,7 @@ PF_DEV void fused_march(const Mesh<R>& m, R* out2, FusedLds<R, TJ, KZ>& L, int k
     __builtin_amdgcn_sched_barrier(0);
     Raw<R> rb = load_raw<R>(bf, ib + 1 <= pmax ? ib + 1 : pmax, plane, boff, col);
     __builtin_amdgcn_sched_barrier(0);
-    Old<R> ob = load_old<R, COMP>(bf, ib >= 1 ? ib - 1 : 0, plane, boff, loff);
+    Old<R> ob = load_old<R, COMP>(bf, ib >= 1 ? ib - 1 : 0, plane, boff, loff, ooff, oloff);
     __builtin_amdgcn_sched_barrier(0);
     Old<R> oa = {};  // the first plane (ib-2) is warm-up only
     L.su[1][0][row][kl] = c0.x;
@@ -166,6 +173,8 @@ PF_DEV void fused_march(const Mesh<R>& m, R* out2, FusedLds<R, TJ, KZ>& L, int k
     L.su[0][1][row][kl] = c1.y;
     L.su[0][2][row][kl] = c1.z;
     L.sid[(ib - 1) & 3][row][kl] = (uint16_t)c1.id;
+    L.sid[ib & 3][row][kl] = 0;  // read (as plane ib-3) before any plane is there
+    L.sid[(ib + 1) & 3][row][kl] = 0;
     __syncthreads();
 
     // stage A: N at plane i, T5/T6 at plane i-1
@@ -246,36 +255,56 @@ PF_DEV void fused_march(const Mesh<R>& m, R* out2, FusedLds<R, TJ, KZ>& L, int k
         const R ol2 = COMP ? lo_decode(oin.o2, oin.ol2) : R(0);
         const bool wall = !FULL && i == 0;
         const V<R> curx = {c0.x, cl0}, cury = {c0.y, cl1}, curz = {c0.z, cl2};
+        const V<R> abx = {ux0_kp, COMP ? above0(cl0) : R(0)};
+        const V<R> aby = {uy0_kp, COMP ? above0(cl1) : R(0)};
+        const V<R> abz = {above0(c0.z), COMP ? above0(cl2) : R(0)};
         V<R> ax, ay, az;
         {
             const R dv = div3<EXACT>(n1_1 - n1_0, sel(ks, xq0.hf, m.fx0), s6 - s6_jm, hy_n, s5 - sel(ks, s5_km, R(0)),
                                      hz_n);
             ax = leap_from<MODE, SPONGE>(curx, oin.o0, ol0, mget(smat, c1.id, M_COEF), dv, sa);
-            const V<R> above = {ux0_kp, COMP ? above0(cl0) : R(0)};
             if (sta && wx && (FULL || i <= m.nx - 3))
-                put_tile<0, MODE>(m, bf, boff, loff, soff, slo, top_xy, ax, curx, above);
+                put_tile<0, MODE>(m, bf, boff, loff, soff, slo, top_xy, ax, curx, abx);
         }
         {
             const R dv = div3<EXACT>(s6 - s6_m, gxa, n2_0_jp - n2_0, uy_g2, s4 - sel(ks, s4_km, R(0)), hz_n);
             ay = leap_from<MODE, SPONGE>(cury, oin.o1, ol1, mget(smat, id_jp, M_COEF), dv, sa);
             if (wall) ay = {R(0), R(0)};
-            const V<R> above = {uy0_kp, COMP ? above0(cl1) : R(0)};
             if (sta && wy && (FULL || i <= m.nx - 2))
-                put_tile<1, MODE>(m, bf, boff, loff, soff, slo, top_xy, ay, cury, above);
+                put_tile<1, MODE>(m, bf, boff, loff, soff, slo, top_xy, ay, cury, aby);
         }
         {
             const R dv = div3<EXACT>(s5 - s5_m, gxa, s4 - s4_jm, hy_n, n3_0_kp - n3_0, uz_g3);
             az = leap_from<MODE, SPONGE>(curz, oin.o2, ol2, mget(smat, ks ? id_kp : c0.id, M_COEF), dv, sa);
             if (wall) az = {R(0), R(0)};
-            const V<R> above = {above0(c0.z), COMP ? above0(cl2) : R(0)};
             if (sta && wz && (FULL || i <= m.nx - 2))
-                put_tile<2, MODE>(m, bf, boff, loff, soff, slo, top_z, az, curz, above);
+                put_tile<2, MODE>(m, bf, boff, loff, soff, slo, top_z, az, curz, abz);
         }
-        oin = load_old<R, COMP>(bf, i + 2 <= omax ? i + 2 : omax, plane, boff, loff);
+        oin = load_old<R, COMP>(bf, i + 2 <= omax ? i + 2 : omax, plane, boff, loff, ooff, oloff);
+        // level n+1 of plane i as stage B reads it: as stored; on the
+        // z = max faces the face value the lane below computes (a Mur map of
+        // its unrounded stage-I value, as put_tile stores it), 0 on the
+        // never-written uz(i, j, nz-1); the wall line takes w(tt+1)
         const bool pin = valid && (FULL || (i >= 0 && i < m.nx));
-        const V<R> b1x = as_stored<MODE>(ax, pin), b1y = as_stored<MODE>(ay, pin);
-        V<R> b1z = as_stored<MODE>(az, pin);
-        if (!FULL && i == 0 && k == 0) b1z = {valid ? w1 : R(0), R(0)};
+        V<R> b1x, b1y, b1z;
+        if constexpr (!FULL) {
+            const V<R> fx = mur_map<MODE>(curx, m.mur[9], m.mur_lo[9], ax, abx);
+            const V<R> fy = mur_map<MODE>(cury, m.mur[10], m.mur_lo[10], ay, aby);
+            const V<R> fz = mur_map<MODE>(curz, m.mur[11], m.mur_lo[11], az, abz);
+            // the lane shifts run on every lane, outside any condition
+            const V<R> gx = {below0(fx.h), COMP ? below0(fx.l) : R(0)};
+            const V<R> gy = {below0(fy.h), COMP ? below0(fy.l) : R(0)};
+            const V<R> gz = {below0(fz.h), COMP ? below0(fz.l) : R(0)};
+            const bool zx = k == m.nz - 1, zz = k == m.nz - 2;
+            b1x = as_stored<MODE>(V<R>{zx ? gx.h : ax.h, zx ? gx.l : ax.l}, pin);
+            b1y = as_stored<MODE>(V<R>{zx ? gy.h : ay.h, zx ? gy.l : ay.l}, pin);
+            b1z = as_stored<MODE>(V<R>{zz ? gz.h : az.h, zz ? gz.l : az.l}, pin && !zx);
+            if (i == 0 && k == 0) b1z = {valid ? w1 : R(0), R(0)};
+        } else {
+            b1x = as_stored<MODE>(ax, pin);
+            b1y = as_stored<MODE>(ay, pin);
+            b1z = as_stored<MODE>(az, pin);
+        }
         L.sb[H][0][row][kl] = b1x.h;
         L.sb[H][1][row][kl] = b1y.h;
         L.sb[H][2][row][kl] = b1z.h;
@@ -340,6 +369,9 @@ PF_DEV void fused_march(const Mesh<R>& m, R* out2, FusedLds<R, TJ, KZ>& L, int k
             const R sb_ = SPONGE ? (xb0.dp + dpyz) + dpz_k : R(0);
             const R gxb = sel(ks, xb0.hs, m.sx0);
             const bool stb = p >= ib && p < ie && wb && (FULL || p <= m.nx - 4);
+            const V<R> bax = {vx0_kp, COMP ? above0(b0x.l) : R(0)};
+            const V<R> bay = {vy0_kp, COMP ? above0(b0y.l) : R(0)};
+            const V<R> baz = {above0(b0z.h), COMP ? above0(b0z.l) : R(0)};
             const unsigned soffb = uni((unsigned)p * plane * (unsigned)sizeof(R));
             const unsigned slob = uni((unsigned)p * plane * (unsigned)sizeof(lo_t));
             const bool wallb = !FULL && p == 0;
@@ -347,19 +379,19 @@ PF_DEV void fused_march(const Mesh<R>& m, R* out2, FusedLds<R, TJ, KZ>& L, int k
                 const R dv = div3<EXACT>(bn1_1 - bn1_0, sel(ks, xb0.hf, m.fx0), t6 - t6_jm, hy_n,
                                          t5 - sel(ks, t5_km, R(0)), hz_n);
                 const V<R> r = leap_from<MODE, SPONGE>(b0x, px.h, px.l, mget(smat, b1id, M_COEF), dv, sb_);
-                if (stb) put_tile<0, MODE>(m, bf2, boff, loff, soffb, slob, false, r, b0x, r);
+                if (stb && wx) put_tile<0, MODE>(m, bf2, boff, loff, soffb, slob, top_xy, r, b0x, bax);
             }
             {
                 const R dv = div3<EXACT>(t6 - bs6_m, gxb, bn2_0_jp - bn2_0, uy_g2, t4 - sel(ks, t4_km, R(0)), hz_n);
                 V<R> r = leap_from<MODE, SPONGE>(b0y, py.h, py.l, mget(smat, bid_jp, M_COEF), dv, sb_);
                 if (wallb) r = {R(0), R(0)};
-                if (stb) put_tile<1, MODE>(m, bf2, boff, loff, soffb, slob, false, r, b0y, r);
+                if (stb && wy) put_tile<1, MODE>(m, bf2, boff, loff, soffb, slob, top_xy, r, b0y, bay);
             }
             {
                 const R dv = div3<EXACT>(t5 - bs5_m, gxb, t4 - t4_jm, hy_n, bn3_0_kp - bn3_0, uz_g3);
                 V<R> r = leap_from<MODE, SPONGE>(b0z, pz.h, pz.l, mget(smat, ks ? bid_kp : b0id, M_COEF), dv, sb_);
                 if (wallb) r = {R(0), R(0)};
-                if (stb) put_tile<2, MODE>(m, bf2, boff, loff, soffb, slob, false, r, b0z, r);
+                if (stb && wz) put_tile<2, MODE>(m, bf2, boff, loff, soffb, slob, top_z, r, b0z, baz);
             }
         }
 
@@ -406,7 +438,7 @@ inline TileMap fused_map(const pf_ctx* c, int tj, int kz, int xchunk) {
         xchunk = (int)((c->nx + want - 1) / want);
         if (xchunk < 32) xchunk = 32;
     }
-    if (xchunk > XT2 - 5) xchunk = XT2 - 5;
+    if (xchunk > XT2 - 6) xchunk = XT2 - 6;
     if (xchunk > c->nx) xchunk = c->nx;
     tm.xchunk = xchunk;
     tm.nxc = (c->nx + xchunk - 1) / xchunk;
@@ -432,7 +464,7 @@ PF_DEV void fused_tile(const Mesh<R>& m, R* out2, const TileMap& tm, int t, Fuse
     const int tid = threadIdx.y * 64 + threadIdx.x;
 
     for (int e = tid; e < m.nmat * MAT_STRIDE; e += TJ * KZ) L.smat[e] = m.mat[e];
-    for (int e = tid; e < ie - ib + 5; e += TJ * KZ) {
+    for (int e = tid; e < ie - ib + 6; e += TJ * KZ) {
         const int p = ib - 2 + e;
         XPlane<R> x;
         x.hs = (p >= 1 && p <= m.nx - 2) ? m.sx[p - 1] : R(1);
@@ -458,27 +490,18 @@ __global__ __launch_bounds__(TJ * KZ, 1) void k_sweep_fused(Mesh<R> m, R* out2, 
 
 // ----------------------------------------------------------- band ----
 // Cells the two-step pass leaves to the one-step code, z fastest:
-//   K: k in {nz-3, nz-2}, all i, j (the z = max faces above them included)
-//   J: j in {1, ny-3, ny-2}, k <= nz-4, all i
-//   I: i in {nx-3, nx-2}, 2 <= j <= ny-4, k <= nz-4
+//   J: j in {1, ny-3, ny-2}, k <= nz-2, i <= nx-2
+//   I: i in {nx-3, nx-2}, 2 <= j <= ny-4, k <= nz-2
 // (no component has a non-face cell at i = nx-1, j = 0, j = ny-1 or
-// k = nz-1).  Needs nx, ny, nz >= 8.
+// k = nz-1; the z = max faces above k = nz-2 / nz-3 come with their cell).
+// Needs nx, ny, nz >= 8.
 inline long long band_total(int nx, int ny, int nz) {
-    return 2LL * nx * ny + 3LL * nx * (nz - 3) + 2LL * (ny - 5) * (nz - 3);
+    return 3LL * (nx - 1) * (nz - 1) + 2LL * (ny - 5) * (nz - 1);
 }
 
 PF_DEV void band_cell(int nx, int ny, int nz, long long t, int& i, int& j, int& k) {
-    long long n = 2LL * nx * ny;
-    if (t < n) {
-        k = nz - 3 + (int)(t % 2);
-        const long long r = t / 2;
-        j = (int)(r % ny);
-        i = (int)(r / ny);
-        return;
-    }
-    t -= n;
-    const int kz = nz - 3;
-    n = 3LL * nx * kz;
+    const int kz = nz - 1;
+    const long long n = 3LL * (nx - 1) * kz;
     if (t < n) {
         k = (int)(t % kz);
         const long long r = t / kz;

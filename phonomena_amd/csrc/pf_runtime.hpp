@@ -76,8 +76,9 @@ Mesh<R> make_view(const pf_ctx* c, int cur, int old, int out);
 // Issue one step (sweep, faces, probe) on stream s; tt = *base + off (or
 // off when base is null).  ev: 3 events recorded around the launches, or null.
 int step_once(pf_ctx* c, hipStream_t s, const long long* base, long long off, hipEvent_t* ev);
-// Issue steps tt and tt+1 with the two-step sweep (PF_KERNEL_TILED2).
-int step_pair(pf_ctx* c, hipStream_t s, const long long* base, long long off);
+// Issue steps tt and tt+1 with the two-step sweep (PF_KERNEL_TILED2); ev as
+// for step_once (the middle event follows the two-step pass).
+int step_pair(pf_ctx* c, hipStream_t s, const long long* base, long long off, hipEvent_t* ev);
 
 }  // namespace pf
 

@@ -120,6 +120,10 @@ PF_DEV unsigned uni(unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlan
 struct Bufs {
     Rsrc cur, old, nxt, ids;  // level n (read), n-1 (read), n+1 (written)
     unsigned ab, lb;
+    // Lane offset past the end of a level (0: none): a buffer load there
+    // returns 0 without reading memory, which lanes whose value is never
+    // used take.  Only when a level is < 2 GiB, so no offset sum wraps.
+    unsigned far;
 };
 template <class R, bool COMP> PF_DEV Bufs make_bufs(const Mesh<R>& m) {
     const long long n = (long long)m.nx * m.plane;
@@ -131,6 +135,7 @@ template <class R, bool COMP> PF_DEV Bufs make_bufs(const Mesh<R>& m) {
     b.ids = arr_rsrc(m.ids, n);
     b.ab = uni((unsigned)(n * (long long)sizeof(R)));
     b.lb = uni((unsigned)(n * (long long)sizeof(lo_t)));
+    b.far = lvl < (1LL << 31) ? 0x80000000u : 0u;
     return b;
 }
 
@@ -177,7 +182,9 @@ PF_DEV Col<R> settle(const Raw<R>& r, bool valid, bool src, R w) {
 
 // Values the displacement update of a plane reads at the node itself: the
 // previous level (hi, and the stored 16-bit lo in FP32C) and the current
-// level's stored lo part (decoded at use, against their hi).
+// level's stored lo part (decoded at use, against their hi).  The previous
+// level is read at lane offsets ooff / oloff (the far offset for lanes that
+// do not use it).
 template <class R>
 struct Old {
     R o0, o1, o2;
@@ -185,17 +192,18 @@ struct Old {
 };
 
 template <class R, bool COMP>
-PF_DEV Old<R> load_old(const Bufs& bf, int i, unsigned plane, unsigned boff, unsigned loff) {
+PF_DEV Old<R> load_old(const Bufs& bf, int i, unsigned plane, unsigned boff, unsigned loff, unsigned ooff,
+                       unsigned oloff) {
     const unsigned soff = uni((unsigned)i * plane * (unsigned)sizeof(R));
     Old<R> o;
-    o.o0 = bld(bf.old, boff, soff, R(0));
-    o.o1 = bld(bf.old, boff, soff + bf.ab, R(0));
-    o.o2 = bld(bf.old, boff, soff + 2 * bf.ab, R(0));
+    o.o0 = bld(bf.old, ooff, soff, R(0));
+    o.o1 = bld(bf.old, ooff, soff + bf.ab, R(0));
+    o.o2 = bld(bf.old, ooff, soff + 2 * bf.ab, R(0));
     if constexpr (COMP) {
         const unsigned sl = uni((unsigned)i * plane * (unsigned)sizeof(lo_t)) + 3 * bf.ab;
-        o.ol0 = bld_lo(bf.old, loff, sl);
-        o.ol1 = bld_lo(bf.old, loff, sl + bf.lb);
-        o.ol2 = bld_lo(bf.old, loff, sl + 2 * bf.lb);
+        o.ol0 = bld_lo(bf.old, oloff, sl);
+        o.ol1 = bld_lo(bf.old, oloff, sl + bf.lb);
+        o.ol2 = bld_lo(bf.old, oloff, sl + 2 * bf.lb);
         o.cl0 = bld_lo(bf.cur, loff, sl);
         o.cl1 = bld_lo(bf.cur, loff, sl + bf.lb);
         o.cl2 = bld_lo(bf.cur, loff, sl + 2 * bf.lb);
@@ -321,6 +329,9 @@ PF_DEV void tiled_march(const Mesh<R>& m, TiledLds<R, TJ, KZ>& L, int kl, int ro
     const bool top_z = !FULL && k == m.nz - 3;
 
     const Bufs bf = make_bufs<R, COMP>(m);
+    // only writer lanes use the previous level
+    const unsigned ooff = writer || !bf.far ? boff : bf.far;
+    const unsigned oloff = writer || !bf.far ? loff : bf.far;
     // prologue: plane ib-1 (warm-up, previous-plane buffer 1), then the
     // loads in flight for the first plane of each parity.  Loads of planes
     // the chunk does not need are clamped to plane ie (or nx-1), which the
@@ -335,7 +346,7 @@ PF_DEV void tiled_march(const Mesh<R>& m, TiledLds<R, TJ, KZ>& L, int kl, int ro
     __builtin_amdgcn_sched_barrier(0);
     Raw<R> rb = load_raw<R>(bf, ib + 1 <= pmax ? ib + 1 : pmax, plane, boff, col);
     __builtin_amdgcn_sched_barrier(0);
-    Old<R> ob = load_old<R, COMP>(bf, ib, plane, boff, loff);
+    Old<R> ob = load_old<R, COMP>(bf, ib, plane, boff, loff, ooff, oloff);
     __builtin_amdgcn_sched_barrier(0);
     Old<R> oa = {};  // the warm-up plane loads plane ib+1 into it
     L.su[1][0][row][kl] = c0.x;
@@ -456,7 +467,7 @@ PF_DEV void tiled_march(const Mesh<R>& m, TiledLds<R, TJ, KZ>& L, int kl, int ro
                     put_tile<2, MODE>(m, bf, boff, loff, soff, slo, top_z, s0, cur, above);
             }
         }
-        oin = load_old<R, COMP>(bf, i + 2 < ie ? i + 2 : ie - 1, plane, boff, loff);
+        oin = load_old<R, COMP>(bf, i + 2 < ie ? i + 2 : ie - 1, plane, boff, loff, ooff, oloff);
         c0 = c1;
         n1_0 = n1_1;
         n2_0 = n2_1;

@@ -1,0 +1,98 @@
+"""The two-step tiled sweep (levels n+1 and n+2 per pass) against the
+one-step tiled sweep: the same per-value formulas on the same operands, so
+identical bits in FP32 and FP32C for every field, both time levels and the
+probe frames, over even and odd step counts, both tile shapes, several x
+chunkings, with and without the sponge."""
+import numpy as np
+import pytest
+
+from helpers import CASES, golden, spec_from, spec_from_golden, uniform_crystal
+from phonomena_amd import fdtd, problem
+
+pytestmark = pytest.mark.gpu
+FIELDS = ["ux", "uy", "uz", "ux_old", "uy_old", "uz_old"]
+
+
+def _run(spec, precision, kernel, chunks, **tiles):
+    ctx = fdtd.Context(spec, precision=precision, kernel=kernel, **tiles)
+    for n in chunks:
+        ctx.step(n)
+    ctx.sync()
+    out = {f: ctx.get(f) for f in FIELDS}
+    out["probes"] = ctx.probes()
+    ctx.close()
+    return out
+
+
+def _same(a, b):
+    for f in a:
+        assert np.array_equal(a[f], b[f]), f
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("precision", ["fp32", "fp32c"])
+def test_pair_equals_tiled_on_golden_meshes(name, precision):
+    spec = spec_from_golden(golden(name))
+    if min(spec.nodes) < 8:
+        pytest.skip("the two-step sweep needs >= 8 nodes per axis")
+    steps = spec.wave.size
+    a = _run(spec, precision, "tiled", [steps])
+    b = _run(spec, precision, "tiled2", [steps])
+    _same(a, b)
+
+
+@pytest.mark.parametrize("tile_j,tile_z,tile_x", [(16, 64, 0), (32, 32, 0), (16, 64, 5), (32, 32, 9),
+                                                 (32, 32, 32), (16, 64, 250)])
+@pytest.mark.parametrize("precision", ["fp32", "fp32c"])
+def test_pair_shapes_on_crystal(tile_j, tile_z, tile_x, precision):
+    x, y, z, ids, rows, dt, wave = uniform_crystal(
+        (70, 45, 131), 1.0, incl=(19300.0, 2.0e11, 1.6e11, 14, 5), steps=121, seed=5)
+    spec = spec_from(x, y, z, ids, rows, dt, wave, probe=(0, 3, 24, 1, 2, 23), every=7)
+    a = _run(spec, precision, "tiled", [121])
+    # an odd count: 60 pairs and one single step
+    b = _run(spec, precision, "tiled2", [121], tile_j=tile_j, tile_z=tile_z, tile_x=tile_x)
+    _same(a, b)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp32c"])
+def test_pair_single_steps_between_pairs(precision):
+    # single steps shift the pair phase and the level rotation
+    x, y, z, ids, rows, dt, wave = uniform_crystal(
+        (40, 33, 50), 1.0, incl=(19300.0, 2.0e11, 1.6e11, 10, 3), steps=60, seed=2)
+    spec = spec_from(x, y, z, ids, rows, dt, wave, probe=(1, 2, 15, 0, 3, 11), every=3)
+    a = _run(spec, precision, "tiled", [60])
+    b = _run(spec, precision, "tiled2", [1, 6, 3, 1, 1, 17, 2, 29])
+    _same(a, b)
+
+
+@pytest.mark.parametrize("tile_j,tile_z", [(32, 32), (16, 64)])
+@pytest.mark.parametrize("precision", ["fp32", "fp32c"])
+def test_pair_interior_tiles_on_larger_mesh(tile_j, tile_z, precision):
+    # big enough that most blocks take the boundary-free (interior) path
+    x, y, z, ids, rows, dt, wave = uniform_crystal(
+        (120, 100, 140), 1.0, incl=(19300.0, 2.0e11, 1.6e11, 20, 7), steps=40, seed=11)
+    spec = spec_from(x, y, z, ids, rows, dt, wave, probe=(2, 5, 20, 3, 4, 20), every=4)
+    a = _run(spec, precision, "tiled", [40])
+    b = _run(spec, precision, "tiled2", [40], tile_j=tile_j, tile_z=tile_z)
+    _same(a, b)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp32c"])
+def test_pair_sponge_equals_tiled(precision):
+    x, y, z, ids, rows, dt, wave = uniform_crystal(
+        (40, 40, 40), 1.0, incl=(1.2, 1.4e5, 0.0, 10, 3), steps=151, seed=9)
+    sponge = problem.sponge_tables((41, 41, 41), {"width": 8, "strength": 0.15})
+    spec = spec_from(x, y, z, ids, rows, dt, wave, sponge=sponge)
+    a = _run(spec, precision, "tiled", [151])
+    b = _run(spec, precision, "tiled2", [151])
+    _same(a, b)
+
+
+def test_pair_rejects_fp64_and_other_tiles():
+    x, y, z, ids, rows, dt, wave = uniform_crystal(
+        (20, 20, 20), 1.0, incl=(19300.0, 2.0e11, 1.6e11, 6, 2), steps=4, seed=1)
+    spec = spec_from(x, y, z, ids, rows, dt, wave)
+    with pytest.raises(fdtd.FdtdError, match="FP32"):
+        fdtd.Context(spec, precision="fp64", kernel="tiled2")
+    with pytest.raises(fdtd.FdtdError, match="16 x 64 or 32 x 32"):
+        fdtd.Context(spec, precision="fp32", kernel="tiled2", tile_j=8, tile_z=64)
