@@ -100,14 +100,18 @@ PF_DEV R div3_uz_surface(R d1, R g1, R d2, R g2, R t3k1, R t3k0, R fz0) {
 
 // ---- leapfrog ---------------------------------------------------------------
 // u_new = 2 u - u_old + coef * div (base_solver.py:441-443).  With a damping
-// term s > 0 (sponge extension): ((2u - (1-s) u_old) + coef*div) / (1+s).
+// term s > 0 (sponge extension): ((2u - (1-s) u_old) + coef*div) / (1+s);
+// in FP32 the division is a product with 1/(1+s), one quotient per node
+// shared by its three components (the same expression in every call, so
+// the compiler forms it once).  At s = 0 both forms give the bits of the
+// undamped update (1 * x and x / 1 are exact).
 template <bool EXACT, bool SPONGE, class R>
 PF_DEV R leap(R u, R uo, R coef, R dv, R s) {
     if constexpr (EXACT) {
         if constexpr (SPONGE) return (2 * u - (1 - s) * uo + coef * dv) / (1 + s);
         else return 2 * u - uo + coef * dv;
     } else {
-        if constexpr (SPONGE) return mad(coef, dv, 2 * u - (1 - s) * uo) / (1 + s);
+        if constexpr (SPONGE) return mad(coef, dv, 2 * u - (1 - s) * uo) * (R(1) / (1 + s));
         else return mad(coef, dv, 2 * u - uo);
     }
 }
