@@ -10,7 +10,7 @@ leaving the final fields in ``self.g`` (as base_solver.py:224-280 does);
 
 cfg keys added to the reference's {wave, wave_args, write_mode}:
   precision     'fp64' (reference arithmetic) | 'fp32' | 'fp32c' (compensated)
-  kernel        'auto' | 'naive' | 'tiled'
+  kernel        'auto' | 'naive' | 'tiled' | 'strip' | 'tiled2' (two steps per pass)
   chunk         steps issued between cancel checks / progress updates
   probe         None | [i0, si, ni, j0, sj, nj] | {'stride': s} | {'count': n}
   record_every  probe frame every k steps (frames land in ``self.probes``)
