@@ -152,6 +152,35 @@ def bench_plain(args):
     return rec
 
 
+def bench_setup(workload="plain", precision="fp32"):
+    """Set-up of the plain workload's mesh: the material map built on the
+    host (the reference's Grid.inclusionIndices + Material.setConstants
+    loop, restated in phonomena_amd.simulation) and uploaded, against the
+    map rasterized on the device from the inclusion list.  Each time covers
+    lattice + material set-up and context creation (allocation, tables,
+    map); the two maps are compared byte for byte."""
+    from phonomena_amd import fdtd, problem
+    from phonomena_amd.simulation import lattice
+    wl = WORKLOADS[workload]
+    out = {"workload": "%dx%dx%d-cell crystal, lattice a=%d r=%d, seed %d" % (wl["cells"] + (wl["a"], wl["r"], wl["seed"]))}
+    maps = []
+    for mode in ("host", "device"):
+        t0 = time.perf_counter()
+        g, m = lattice.crystal(wl["cells"], wl["a"], wl["r"], wl["seed"], host_ids=(mode == "host"))
+        t1 = time.perf_counter()
+        spec = problem.mesh_spec(g, m, np.zeros(2), {}, device_ids=(mode == "device"))
+        ctx = fdtd.Context(spec, precision=precision)
+        ctx.sync()
+        t2 = time.perf_counter()
+        maps.append(ctx.ids())
+        ctx.close()
+        out[mode] = {"lattice_material_s": t1 - t0, "context_s": t2 - t1, "total_s": t2 - t0}
+    out["inclusions"] = int(len(g.targets))
+    out["maps_equal"] = bool(np.array_equal(maps[0], maps[1]))
+    out["speedup"] = out["host"]["total_s"] / out["device"]["total_s"]
+    return out
+
+
 def bench_batched(steps, precisions=("fp32c", "fp32"), n=64, seed=64, trial_steps=2):
     """The 64-crystal mix: the plain single-instance loop (one context at a
     time, pf_step) against one batch of all of them (autotuned, graph-
@@ -281,6 +310,7 @@ def main():
     ap.add_argument("--precision-steps", type=int, default=10000,
                     help="steps of the FP32C/FP32 vs FP64 precision case (0 = skip)")
     ap.add_argument("--serve-jobs", type=int, default=48, help="jobs in the served stream")
+    ap.add_argument("--no-setup", action="store_true", help="skip the mesh set-up (material map) leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design "
@@ -299,6 +329,8 @@ def main():
             rec["served"] = bench_served(args.serve_jobs, rec["batched"]["value"] / mean_work)
     if args.precision_steps > 0:
         rec["precision_case"] = bench_precision(args.precision_steps)
+    if not args.no_setup:
+        rec["setup"] = bench_setup(args.workload)
     rec["cpu_baseline"] = None if args.no_cpu else cpu_baseline()
     print(json.dumps(rec), flush=True)
 

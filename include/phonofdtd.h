@@ -75,7 +75,9 @@ typedef struct pf_desc {
     /* n_materials rows of 13: C00 C01 C02 C10 C11 C12 C20 C21 C22 (the normal
        3x3 block of the 6x6 stiffness), C33 C44 C55 (shear diagonal), rho.   */
     const double *materials;
-    const uint8_t *ids;        /* nx*ny*nz material index per node             */
+    const uint8_t *ids;        /* nx*ny*nz material index per node, or NULL:
+                                  rasterized on the device from the inclusion
+                                  list below                                   */
     double dt;
     double dt2;                /* dt**2 exactly as the host computed it         */
     const double *mur;         /* 12 Mur coefficients: faces x=max, y=0, y=max,
@@ -97,6 +99,18 @@ typedef struct pf_desc {
                                   row per wavefront) or 32 (two rows per
                                   wavefront); 0 = the one with fewer idle
                                   lanes for this mesh                          */
+    /* Material map from an inclusion list (used when ids is NULL), the
+       device form of Grid.inclusionIndices + Material.setConstants
+       (grid.py:158-175, material.py:55-63): node (i, j, k) takes index 1
+       when some inclusion m has sqrt((x[i]-tx)^2 + (y[j]-ty)^2) < r (FP64,
+       the reference's order) and k < K_m, else index 0. */
+    const double *node_x;      /* nx node coordinates along x                  */
+    const double *node_y;      /* ny node coordinates along y                  */
+    int32_t n_targets;         /* inclusions                                   */
+    const double *targets;     /* n_targets x 4: tx, ty, r (the reference's
+                                  float32 target record, widened), K_m = z
+                                  nodes covered (0..nz, from the reference's
+                                  z test)                                      */
 } pf_desc;
 
 /* Library / device identification. */
@@ -119,6 +133,9 @@ int pf_steps_done(pf_ctx *ctx, int64_t *out);
 
 /* Fields in reference shapes, converted to/from double. */
 int pf_get_field(pf_ctx *ctx, int32_t which, double *host, int64_t count);
+/* The per-node material index map (nx*ny*nz bytes, z fastest), as uploaded
+   or rasterized; Material.ids of material.py:48-63. */
+int pf_get_ids(pf_ctx *ctx, uint8_t *host, int64_t count);
 int pf_set_field(pf_ctx *ctx, int32_t which, const double *host, int64_t count);
 
 /* Probe frames recorded so far (oldest first, at most capacity). */

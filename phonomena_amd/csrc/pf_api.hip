@@ -15,6 +15,7 @@
 #include "pf_runtime.hpp"
 #include "pf_strip.hpp"
 #include "pf_fused.hpp"
+#include "pf_raster.hpp"
 #include "pf_tiled.hpp"
 
 using namespace pf;
@@ -85,7 +86,18 @@ int check_desc(const pf_desc* d) {
     if (d->kernel < PF_KERNEL_AUTO || d->kernel > PF_KERNEL_TILED2) return fail(PF_ERR_ARG, "bad kernel %d", d->kernel);
     if (d->n_materials < 1 || d->n_materials > 256) return fail(PF_ERR_ARG, "n_materials must be 1..256");
     if (!d->fdx || !d->fdy || !d->fdz || !d->sdx || !d->sdy || !d->sdz) return fail(PF_ERR_ARG, "spacing table missing");
-    if (!d->materials || !d->ids) return fail(PF_ERR_ARG, "material table or ids missing");
+    if (!d->materials) return fail(PF_ERR_ARG, "material table missing");
+    if (!d->ids) {
+        if (!d->node_x || !d->node_y || d->n_targets < 0 || (d->n_targets > 0 && !d->targets))
+            return fail(PF_ERR_ARG, "ids missing and no inclusion list to rasterize");
+        if (d->n_materials < 2 && d->n_targets > 0)
+            return fail(PF_ERR_ARG, "inclusions need a secondary material (n_materials >= 2)");
+        for (int m = 0; m < d->n_targets; m++) {
+            const double k = d->targets[4 * m + 3];
+            if (!(k >= 0 && k <= d->nz && k == (double)(int)k))
+                return fail(PF_ERR_ARG, "target %d covers %g z nodes (must be an integer in 0..nz)", m, k);
+        }
+    }
     if (!d->mur) return fail(PF_ERR_ARG, "Mur coefficients missing");
     if (!d->wave || d->wave_len < 0) return fail(PF_ERR_ARG, "source table missing");
     if (!(d->dt > 0) || !(d->dt2 > 0)) return fail(PF_ERR_ARG, "dt must be > 0");
@@ -98,8 +110,10 @@ int check_desc(const pf_desc* d) {
     for (int r = 0; r < d->n_materials; r++)
         if (!(d->materials[r * 13 + 12] > 0)) return fail(PF_ERR_ARG, "material %d has rho <= 0", r);
     const long long nn = (long long)d->nx * d->ny * d->nz;
-    for (long long t = 0; t < nn; t++)
-        if (d->ids[t] >= d->n_materials) return fail(PF_ERR_ARG, "ids[%lld] = %d out of range", t, (int)d->ids[t]);
+    if (d->ids)
+        for (long long t = 0; t < nn; t++)
+            if (d->ids[t] >= d->n_materials)
+                return fail(PF_ERR_ARG, "ids[%lld] = %d out of range", t, (int)d->ids[t]);
     if (d->probe[2] > 0) {
         const int* p = d->probe;
         if (p[1] < 1 || p[4] < 1 || p[5] < 1 || p[0] < 0 || p[3] < 0)
@@ -148,6 +162,47 @@ extern "C" int pf_create(const pf_desc* d, pf_ctx** out) {
     }
     *out = c;
     return PF_OK;
+}
+
+// Material map of the context from the descriptor's inclusion list.
+static int rasterize(pf_ctx* c, const pf_desc* d) {
+    double *x = nullptr, *y = nullptr;
+    RasterTarget* tg = nullptr;
+    int* kmax = nullptr;
+    const long long ncols = (long long)d->nx * d->ny;
+    const int nt = d->n_targets;
+    auto cleanup = [&]() {
+        if (x) (void)hipFree(x);
+        if (y) (void)hipFree(y);
+        if (tg) (void)hipFree(tg);
+        if (kmax) (void)hipFree(kmax);
+    };
+    int rc = PF_OK;
+    do {
+        std::vector<RasterTarget> h((size_t)(nt > 0 ? nt : 1));
+        for (int m = 0; m < nt; m++)
+            h[(size_t)m] = {d->targets[4 * m], d->targets[4 * m + 1], d->targets[4 * m + 2], d->targets[4 * m + 3]};
+        hipError_t e;
+        if ((e = hipMalloc((void**)&x, (size_t)d->nx * sizeof(double))) != hipSuccess ||
+            (e = hipMalloc((void**)&y, (size_t)d->ny * sizeof(double))) != hipSuccess ||
+            (e = hipMalloc((void**)&tg, h.size() * sizeof(RasterTarget))) != hipSuccess ||
+            (e = hipMalloc((void**)&kmax, (size_t)ncols * sizeof(int))) != hipSuccess ||
+            (e = hipMemcpy(x, d->node_x, (size_t)d->nx * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(y, d->node_y, (size_t)d->ny * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(tg, h.data(), h.size() * sizeof(RasterTarget), hipMemcpyHostToDevice)) != hipSuccess) {
+            rc = fail(PF_ERR_HIP, "rasterize: %s", hipGetErrorString(e));
+            break;
+        }
+        hipLaunchKernelGGL(k_raster_cols, dim3((unsigned)((ncols + 255) / 256)), dim3(256), 0, c->stream, x, y, d->nx,
+                           d->ny, tg, nt, kmax);
+        const long long n4 = ncols * ((d->nz + 3) / 4);
+        hipLaunchKernelGGL(k_raster_fill, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, c->stream, kmax, ncols,
+                           d->nz, c->ids);
+        if ((e = hipGetLastError()) != hipSuccess || (e = hipStreamSynchronize(c->stream)) != hipSuccess)
+            rc = fail(PF_ERR_HIP, "rasterize: %s", hipGetErrorString(e));
+    } while (0);
+    cleanup();
+    return rc;
 }
 
 int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
@@ -251,7 +306,10 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
     // material index map, in the field layout (rows unpadded: one copy; a
     // row-by-row 2-D copy took ~0.3 s per million nodes)
     HIPCHK(hipMalloc((void**)&c->ids, (size_t)c->cells));
-    if (c->pz == d->nz) {
+    if (!d->ids) {
+        int rc = rasterize(c, d);
+        if (rc) return rc;
+    } else if (c->pz == d->nz) {
         HIPCHK(hipMemcpy(c->ids, d->ids, (size_t)c->cells, hipMemcpyHostToDevice));
     } else {
         HIPCHK(hipMemset(c->ids, 0, (size_t)c->cells));
@@ -550,6 +608,15 @@ static int ensure_scratch(pf_ctx* c, size_t bytes) {
     c->scratch_bytes = 0;
     HIPCHK(hipMalloc(&c->scratch, bytes));
     c->scratch_bytes = bytes;
+    return PF_OK;
+}
+
+extern "C" int pf_get_ids(pf_ctx* c, uint8_t* host, int64_t count) {
+    if (!c || !host) return fail(PF_ERR_ARG, "NULL argument");
+    if (count != c->cells) return fail(PF_ERR_ARG, "the map has %lld nodes, got %lld", c->cells, (long long)count);
+    if (c->batch) HIPCHK(hipStreamSynchronize(c->batch_stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(host, c->ids, (size_t)c->cells, hipMemcpyDeviceToHost));
     return PF_OK;
 }
 

@@ -48,6 +48,8 @@ class PfDesc(ctypes.Structure):
         ("wave", ctypes.c_void_p), ("wave_len", ctypes.c_int64),
         ("probe", ctypes.c_int32 * 6), ("record_every", ctypes.c_int32), ("probe_capacity", ctypes.c_int32),
         ("tile_j", ctypes.c_int32), ("tile_x", ctypes.c_int32), ("tile_z", ctypes.c_int32),
+        ("node_x", ctypes.c_void_p), ("node_y", ctypes.c_void_p),
+        ("n_targets", ctypes.c_int32), ("targets", ctypes.c_void_p),
     ]
 
 
@@ -79,6 +81,7 @@ def load(path=None):
         "pf_sync": ([V], ctypes.c_int),
         "pf_steps_done": ([V, P(I64)], ctypes.c_int),
         "pf_get_field": ([V, I32, V, I64], ctypes.c_int),
+        "pf_get_ids": ([V, V, I64], ctypes.c_int),
         "pf_set_field": ([V, I32, V, I64], ctypes.c_int),
         "pf_get_probes": ([V, V, I64, P(I64)], ctypes.c_int),
         "pf_energy": ([V, P(D)], ctypes.c_int),
@@ -138,16 +141,25 @@ class MeshSpec:
     (nx, ny, nz); dt and dt2 (= dt**2 as the host computes it); mur: 12
     face coefficients; wave: FP64 source table (its length is the step
     budget); sponge: None or three per-axis damping arrays; probe:
-    (i0, si, ni, j0, sj, nj) or None.
+    (i0, si, ni, j0, sj, nj) or None.  With ids None, raster = (x, y,
+    targets) gives the node coordinates along x and y and the (n, 4)
+    inclusion list (tx, ty, r, z nodes covered) the device rasterizes the
+    material map from (problem.raster_targets).
     """
 
     def __init__(self, nodes, fd, sd, materials, ids, dt, dt2, mur, wave,
-                 sponge=None, probe=None, record_every=1, probe_capacity=1):
+                 sponge=None, probe=None, record_every=1, probe_capacity=1, raster=None):
         self.nodes = tuple(int(n) for n in nodes)
         self.fd = [_c(a, np.float64) for a in fd]
         self.sd = [_c(a, np.float64) for a in sd]
         self.materials = _c(materials, np.float64).reshape(-1, 13)
-        self.ids = _c(ids, np.uint8)
+        self.ids = None if ids is None else _c(ids, np.uint8)
+        self.raster = None
+        if self.ids is None:
+            if raster is None:
+                raise ValueError("ids or an inclusion list to rasterize is required")
+            x, y, tg = raster
+            self.raster = (_c(x, np.float64), _c(y, np.float64), _c(tg, np.float64).reshape(-1, 4))
         self.dt = float(dt)
         self.dt2 = float(dt2)
         self.mur = _c(mur, np.float64).reshape(12)
@@ -157,7 +169,10 @@ class MeshSpec:
         self.record_every = int(record_every)
         self.probe_capacity = int(probe_capacity)
         nx, ny, nz = self.nodes
-        if self.ids.shape != (nx, ny, nz):
+        if self.raster is not None:
+            if self.raster[0].shape != (nx,) or self.raster[1].shape != (ny,):
+                raise ValueError("node coordinates do not match nodes %s" % (self.nodes,))
+        elif self.ids.shape != (nx, ny, nz):
             raise ValueError("ids shape %s != nodes %s" % (self.ids.shape, self.nodes))
 
     @property
@@ -188,7 +203,13 @@ class Context:
         d.fdx, d.fdy, d.fdz = (ptr(a) for a in spec.fd)
         d.sdx, d.sdy, d.sdz = (ptr(a) for a in spec.sd)
         d.materials = ptr(spec.materials)
-        d.ids = ptr(spec.ids)
+        if spec.ids is not None:
+            d.ids = ptr(spec.ids)
+        else:
+            x, y, tg = spec.raster
+            d.node_x, d.node_y = ptr(x), ptr(y)
+            d.n_targets = tg.shape[0]
+            d.targets = ptr(tg)
         d.dt, d.dt2 = spec.dt, spec.dt2
         d.mur = ptr(spec.mur)
         if spec.sponge is not None:
@@ -232,6 +253,12 @@ class Context:
         return v.value
 
     # -- fields
+    def ids(self):
+        """The device's material index map, (nx, ny, nz) uint8."""
+        out = np.empty(self.spec.nodes, dtype=np.uint8)
+        check(self.lib.pf_get_ids(self.handle, out.ctypes.data_as(ctypes.c_void_p), out.size))
+        return out
+
     def get(self, which):
         shape = self._shapes[which.replace("_old", "")]
         out = np.empty(shape, dtype=np.float64)

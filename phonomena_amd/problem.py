@@ -106,18 +106,58 @@ def probe_grid(spec, nodes):
     return p
 
 
-def mesh_spec(g, m, wave, cfg):
-    """MeshSpec of a solver's grid/material/source under its cfg."""
+def raster_targets(g):
+    """(x, y, targets) for the device rasterizer: node coordinates along x
+    and y, and one row (tx, ty, r, K) per inclusion of the grid, K = the
+    number of leading z nodes the inclusion covers.
+
+    K follows the z test of Grid.inclusionIndices (grid.py:158-175): the
+    first target compares the z coordinates with its depth, and every later
+    one compares the index array the previous test returned (the reference
+    rebinds ``z``).  Each selection is a leading run of nodes, so a count
+    describes it."""
+    z = g.z
+    rows = []
+    for t in g.targets:
+        idx = np.array(np.where(z <= t['z'])).flatten()
+        if not np.array_equal(idx, np.arange(idx.size)):
+            raise ValueError("z selection of an inclusion is not a leading run of nodes")
+        rows.append((float(t['x']), float(t['y']), float(t['r']), float(idx.size)))
+        z = idx
+    tg = np.array(rows, dtype=np.float64).reshape(-1, 4)
+    return np.ascontiguousarray(g.x, np.float64), np.ascontiguousarray(g.y, np.float64), tg
+
+
+def raster_id(x, y, k, targets):
+    """Material index of node (x, y, k) under the rasterizer's rule (host)."""
+    for tx, ty, r, kk in targets:
+        X = x - tx
+        Y = y - ty
+        if np.sqrt(np.add(np.square(X), np.square(Y))) < r and k < kk:
+            return 1
+    return 0
+
+
+def mesh_spec(g, m, wave, cfg, device_ids=False):
+    """MeshSpec of a solver's grid/material/source under its cfg.  With
+    device_ids the material map is rasterized on the device from the grid's
+    inclusion list (m.ids is not needed)."""
     nodes = (g.x.size, g.y.size, g.z.size)
     fd, sd = axis_spacings(g)
     rows = m.rows()
     dt = m.dt
     dt2 = dt ** 2
-    mur = mur_coefficients(rows[int(m.ids[0, 0, 0])], dt, fd, sd)
+    raster = None
+    if device_ids:
+        raster = raster_targets(g)
+        id0 = raster_id(raster[0][0], raster[1][0], 0, raster[2])
+    else:
+        id0 = int(m.ids[0, 0, 0])
+    mur = mur_coefficients(rows[id0], dt, fd, sd)
     probe = probe_grid(cfg.get("probe"), nodes)
     every = int(cfg.get("record_every", 10))
     steps = len(wave)
     cap = int(cfg.get("probe_capacity", 0)) or max(1, steps // max(every, 1))
-    return MeshSpec(nodes, fd, sd, rows, m.ids, dt, dt2, mur, wave,
+    return MeshSpec(nodes, fd, sd, rows, None if device_ids else m.ids, dt, dt2, mur, wave,
                     sponge=sponge_tables(nodes, cfg.get("sponge")),
-                    probe=probe, record_every=every, probe_capacity=cap)
+                    probe=probe, record_every=every, probe_capacity=cap, raster=raster)

@@ -9,8 +9,9 @@ the GPU and advances it in chunks of steps.  Between chunks it
   * admits queued jobs as memory frees up: a job joins when its fields fit
     in the HBM budget beside the members already resident, and from the
     next chunk on its work overlaps with theirs in the same launches.
-Host-side preparation of arriving jobs (meshing, material, source table)
-runs while the GPU executes the current chunk.
+Host-side preparation of arriving jobs (lattice, inclusion list, source
+table) runs while the GPU executes the current chunk; the material map is
+rasterized on the device when the job's context is created.
 
 Every member's fields and probes are bit-identical to running the job
 alone (the batch's guarantee), so a served result does not depend on what
@@ -83,17 +84,20 @@ class Server:
     chunk: steps per batched chunk (admission / retirement granularity).
     probe, record_every: the probe grid every job records (surface u_z).
     keep_fields: keep each job's final fields in its Result.
+    device_ids: jobs' material maps are rasterized on the device from their
+    inclusion lists (no host map, no map upload).
     """
 
     def __init__(self, precision="fp32c", hbm_budget=200e9, chunk=200, probe=None, record_every=10,
                  keep_fields=False, tile=None, spec_fn=None, context_factory=None, batch_factory=None,
-                 clock=time.perf_counter):
+                 clock=time.perf_counter, device_ids=True):
         self.precision = precision
         self.hbm_budget = float(hbm_budget)
         self.chunk = int(chunk)
         self.probe = probe if probe is not None else {"count": 32}
         self.record_every = int(record_every)
         self.keep_fields = keep_fields
+        self.device_ids = device_ids
         self.tile = tile
         self.spec_fn = spec_fn or self._spec
         self.context_factory = context_factory or (lambda spec: fdtd.Context(spec, precision=self.precision))
@@ -104,7 +108,8 @@ class Server:
         # the ring holds two chunks of frames, so nothing is overwritten
         # before it is pulled
         cap = 2 * (self.chunk // self.record_every + 1)
-        return sweep.job_spec(job, probe=self.probe, every=self.record_every, capacity=cap)
+        return sweep.job_spec(job, probe=self.probe, every=self.record_every, capacity=cap,
+                              device_ids=self.device_ids)
 
     def run(self, jobs, arrivals):
         """Serve ``jobs`` arriving at ``arrivals`` (seconds from the start).

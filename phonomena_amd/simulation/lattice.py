@@ -65,7 +65,7 @@ def stable_courant(m, courant=0.5, ratio=4.0):
     return courant
 
 
-def crystal(cells, a, r, seed, inclusion=TUNGSTEN, matrix=MATRIX, jitter=0.5, courant=0.5, d=1.0):
+def crystal(cells, a, r, seed, inclusion=TUNGSTEN, matrix=MATRIX, jitter=0.5, courant=0.5, d=1.0, host_ids=True):
     """(Grid, Material) of a crystal of ``cells`` = (Nx, Ny, Nz) cells.
 
     Coordinates are in cell units by default (d = 1), the way the reference's
@@ -75,6 +75,8 @@ def crystal(cells, a, r, seed, inclusion=TUNGSTEN, matrix=MATRIX, jitter=0.5, co
     coordinates such as dx = 5e-5 the reference's free-surface u_z update,
     which adds T3[..,1] without dividing by dz (base_solver.py:516), makes
     the scheme diverge; at unit spacing that term is what was intended.)
+    With host_ids False the material map is not built on the host (the
+    device rasterizes it from the inclusion list).
     """
     g = _grid.Grid()
     g.setSpacing(d)
@@ -86,7 +88,7 @@ def crystal(cells, a, r, seed, inclusion=TUNGSTEN, matrix=MATRIX, jitter=0.5, co
     m.setPrimary("matrix")
     m.setSecondary("inclusion")
     m.c_max = stable_courant(m, courant)
-    m.update()
+    m.update(rasterize=host_ids)
     return g, m
 
 

@@ -47,11 +47,16 @@ class Material:
         self.setPrimary(keys[0])
         self.setSecondary(keys[0])
 
-    def update(self):
-        """Per-node material map and time step (material.py:48-53)."""
+    def update(self, rasterize=True):
+        """Per-node material map and time step (material.py:48-53).  With
+        rasterize False only the time step is set: the map is left to the
+        device rasterizer (problem.mesh_spec(..., device_ids=True))."""
         g = self.grid
-        self.ids = np.zeros((g.x.size, g.y.size, g.z.size), dtype=np.uint8)
         self.setTimeStep()
+        if not rasterize:
+            self.ids = np.zeros((0, 0, 0), dtype=np.uint8)
+            return
+        self.ids = np.zeros((g.x.size, g.y.size, g.z.size), dtype=np.uint8)
         self.setConstants()
 
     def setConstants(self):

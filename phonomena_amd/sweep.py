@@ -22,10 +22,13 @@ INCLUSIONS = {"void": lattice.VOID, "tungsten": lattice.TUNGSTEN}
 
 
 def crystal_spec(cells, a, r, seed, steps, inclusion="tungsten", probe=None, every=10, sponge=None,
-                 kind="gauss", capacity=None):
-    """(MeshSpec, Grid, Material) of one crystal job."""
+                 kind="gauss", capacity=None, device_ids=False):
+    """(MeshSpec, Grid, Material) of one crystal job.  device_ids: the
+    material map is rasterized on the device (the Material's map is left
+    empty)."""
     inc = INCLUSIONS[inclusion] if isinstance(inclusion, str) else inclusion
-    g, m = lattice.crystal(tuple(int(c) for c in cells), float(a), float(r), int(seed), inclusion=inc)
+    g, m = lattice.crystal(tuple(int(c) for c in cells), float(a), float(r), int(seed), inclusion=inc,
+                           host_ids=not device_ids)
     b = BaseSolver(None)
     b.m = m
     b.cfg["wave"], b.cfg["wave_args"] = kind, {"f": F0 * DX_PHYS}
@@ -33,7 +36,7 @@ def crystal_spec(cells, a, r, seed, steps, inclusion="tungsten", probe=None, eve
     cfg = {"probe": probe, "record_every": every, "sponge": sponge}
     if capacity:
         cfg["probe_capacity"] = int(capacity)
-    spec = problem.mesh_spec(g, m, wave, cfg)
+    spec = problem.mesh_spec(g, m, wave, cfg, device_ids=device_ids)
     return spec, g, m
 
 
@@ -57,11 +60,11 @@ def mix(n=64, seed=64, steps=2000):
     return [random_job(rng, steps=(steps, steps)) for _ in range(n)]
 
 
-def job_spec(job, probe=None, every=10, sponge=None, steps=None, capacity=None):
+def job_spec(job, probe=None, every=10, sponge=None, steps=None, capacity=None, device_ids=False):
     """MeshSpec of a job dict (as drawn by ``random_job``)."""
     spec, _, _ = crystal_spec(job["cells"], job["a"], job["r"], job["seed"],
                               job["steps"] if steps is None else steps, inclusion=job["inclusion"],
-                              probe=probe, every=every, sponge=sponge, capacity=capacity)
+                              probe=probe, every=every, sponge=sponge, capacity=capacity, device_ids=device_ids)
     return spec
 
 
