@@ -145,6 +145,17 @@ int pf_get_probes(pf_ctx *ctx, double *host, int64_t cap_frames, int64_t *nframe
 int pf_energy(pf_ctx *ctx, double *out);
 
 /*
+ * Spectral analysis (replaces the numpy transforms of analysis.spectrum,
+ * phonomena/simulation/analysis.py:44-96): a batch of 1-D DFTs along the
+ * contiguous axis in FP64, first nf frequencies,
+ *   out[b, k] = scale * sum_t (in[b, t] * window[t]) exp(-2 pi i k t / n),
+ * in (batch, n) row-major, out (batch, nf).  in_im and window may be NULL
+ * (real input, no window).  Host arrays; runs on the current device.
+ */
+int pf_dft(const double *in_re, const double *in_im, int64_t batch, int64_t n, int64_t nf,
+           const double *window, double scale, double *out_re, double *out_im);
+
+/*
  * Timing: run nsteps steps with HIP events around every sweep and face
  * launch on the context's stream; returns the mean duration of each per
  * step (ms).  With PF_KERNEL_TILED2, sweep = the two-step pass and face =

@@ -15,7 +15,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "lib", "libphonofdtd.so")
-SOURCES = ["pf_api.hip", "pf_batch.hip", "pf_diag.hip"]
+SOURCES = ["pf_api.hip", "pf_batch.hip", "pf_diag.hip", "pf_spectrum.hip"]
 ARCH = os.environ.get("PF_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared",

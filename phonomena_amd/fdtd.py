@@ -95,6 +95,7 @@ def load(path=None):
         "pf_batch_autotune": ([V, I32, P(I32), P(I32), P(I32), P(D)], ctypes.c_int),
         "pf_batch_destroy": ([V], ctypes.c_int),
         "pf_stream_probe": ([I32, I32, I64, I32, P(D), P(D)], ctypes.c_int),
+        "pf_dft": ([V, V, I64, I64, I64, V, D, V, V], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -291,6 +292,30 @@ class Context:
         a, b = ctypes.c_double(), ctypes.c_double()
         check(self.lib.pf_profile(self.handle, int(n), ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+
+def dft(x, nf=None, window=None, scale=1.0):
+    """Batched 1-D DFT along the last axis on the GPU (pf_dft), first nf
+    frequencies: scale * sum_t x[..., t] window[t] exp(-2 pi i k t / n)."""
+    lib = load()
+    x = np.asarray(x)
+    n = x.shape[-1]
+    nf = n if nf is None else int(nf)
+    lead = x.shape[:-1]
+    batch = int(np.prod(lead)) if lead else 1
+    re = np.ascontiguousarray(x.real, dtype=np.float64).reshape(batch, n)
+    im = np.ascontiguousarray(x.imag, dtype=np.float64).reshape(batch, n) if np.iscomplexobj(x) else None
+    w = None if window is None else np.ascontiguousarray(window, dtype=np.float64)
+    if w is not None and w.shape != (n,):
+        raise ValueError("window must have %d values" % n)
+    ore = np.empty((batch, nf))
+    oim = np.empty((batch, nf))
+    vp = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+    for b0 in range(0, batch, 65535):  # one launch covers at most 65535 rows
+        b1 = min(batch, b0 + 65535)
+        check(lib.pf_dft(vp(re[b0:b1]), None if im is None else vp(im[b0:b1]), b1 - b0, n, nf, vp(w),
+                         float(scale), vp(ore[b0:b1]), vp(oim[b0:b1])))
+    return (ore + 1j * oim).reshape(lead + (nf,))
 
 
 class Batch:
