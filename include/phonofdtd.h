@@ -133,6 +133,23 @@ int pf_steps_done(pf_ctx *ctx, int64_t *out);
 
 /* Fields in reference shapes, converted to/from double. */
 int pf_get_field(pf_ctx *ctx, int32_t which, double *host, int64_t count);
+/*
+ * Asynchronous snapshots of the current level, for the HDF output path
+ * (Writer, base_solver.py:72-169, put :256-260): ux, uy, uz in the
+ * reference's shapes as FP64, back to back, in pinned host memory.
+ * pf_snapshot_open allocates `slots` (1..16) staging frames on the device
+ * and in pinned host memory.  pf_snapshot(slot) queues the packing of the
+ * current level on the context's stream and its copy to the host on a
+ * second stream, and returns at once; steps issued afterwards run while the
+ * copy is in flight.  pf_snapshot_wait(slot) waits for the copy and gives
+ * the host frame and the step count it was taken at; the frame stays valid
+ * until pf_snapshot_release(slot) frees the slot for the next snapshot.
+ */
+int pf_snapshot_open(pf_ctx *ctx, int32_t slots);
+int pf_snapshot(pf_ctx *ctx, int32_t slot);
+int pf_snapshot_wait(pf_ctx *ctx, int32_t slot, double **host, int64_t *steps_done);
+int pf_snapshot_release(pf_ctx *ctx, int32_t slot);
+
 /* The per-node material index map (nx*ny*nz bytes, z fastest), as uploaded
    or rasterized; Material.ids of material.py:48-63. */
 int pf_get_ids(pf_ctx *ctx, uint8_t *host, int64_t count);

@@ -385,6 +385,14 @@ void pf::ctx_free(pf_ctx* c) {
     F(c->mat64); F(c->rho64); F(c->fx64); F(c->fy64); F(c->fz64); F(c->sx64); F(c->sy64); F(c->sz64);
     F(c->wave); F(c->d_step); F(c->ring); F(c->d_frame); F(c->e_out); F(c->e_partials);
     F(c->scratch);
+    if (c->snap_stream) (void)hipStreamSynchronize(c->snap_stream);  // no copy in flight
+    F(c->snap_dev);
+    if (c->snap_host) (void)hipHostFree(c->snap_host);
+    for (int i = 0; i < c->snap_slots; i++) {
+        if (c->snap_packed[i]) (void)hipEventDestroy(c->snap_packed[i]);
+        if (c->snap_copied[i]) (void)hipEventDestroy(c->snap_copied[i]);
+    }
+    if (c->snap_stream) (void)hipStreamDestroy(c->snap_stream);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -608,6 +616,86 @@ static int ensure_scratch(pf_ctx* c, size_t bytes) {
     c->scratch_bytes = 0;
     HIPCHK(hipMalloc(&c->scratch, bytes));
     c->scratch_bytes = bytes;
+    return PF_OK;
+}
+
+// ---------------------------------------------------- snapshots ----
+extern "C" int pf_snapshot_open(pf_ctx* c, int32_t slots) {
+    if (!c) return fail(PF_ERR_ARG, "ctx is NULL");
+    if (slots < 1 || slots > PF_SNAP_MAX_SLOTS)
+        return fail(PF_ERR_ARG, "slots must be 1..%d (got %d)", PF_SNAP_MAX_SLOTS, slots);
+    if (c->snap_slots) return fail(PF_ERR_STATE, "snapshots are already open");
+    long long frame = 0;
+    for (int q = 0; q < 3; q++) {
+        int ex, ey, ez;
+        extents(c, q, ex, ey, ez);
+        frame += (long long)ex * ey * ez;
+    }
+    const size_t bytes = (size_t)frame * sizeof(double) * (size_t)slots;
+    HIPCHK(hipMalloc((void**)&c->snap_dev, bytes));
+    HIPCHK(hipHostMalloc((void**)&c->snap_host, bytes, hipHostMallocDefault));
+    HIPCHK(hipStreamCreateWithFlags(&c->snap_stream, hipStreamNonBlocking));
+    for (int i = 0; i < slots; i++) {
+        HIPCHK(hipEventCreateWithFlags(&c->snap_packed[i], hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&c->snap_copied[i], hipEventDisableTiming));
+        c->snap_state[i] = 0;
+    }
+    c->snap_frame = frame;
+    c->snap_slots = slots;
+    return PF_OK;
+}
+
+extern "C" int pf_snapshot(pf_ctx* c, int32_t slot) {
+    if (!c) return fail(PF_ERR_ARG, "ctx is NULL");
+    if (c->batch) return fail(PF_ERR_STATE, "context is stepped by its batch");
+    if (slot < 0 || slot >= c->snap_slots) return fail(PF_ERR_ARG, "no snapshot slot %d", slot);
+    if (c->snap_state[slot] != 0) return fail(PF_ERR_STATE, "snapshot slot %d is in use", slot);
+    double* dev = c->snap_dev + (size_t)slot * (size_t)c->snap_frame;
+    long long off = 0;
+    for (int q = 0; q < 3; q++) {
+        int ex, ey, ez;
+        extents(c, q, ex, ey, ez);
+        const long long n = (long long)ex * ey * ez;
+        const unsigned g = (unsigned)((n + 255) / 256);
+        if (c->esz == 8)
+            hipLaunchKernelGGL((k_pack<double>), dim3(g), dim3(256), 0, c->stream, (const double*)c->lvl[c->cur][q],
+                               (const lo_t*)nullptr, c->plane, c->pz, ex, ey, ez, dev + off, 0, 0.0);
+        else
+            hipLaunchKernelGGL((k_pack<float>), dim3(g), dim3(256), 0, c->stream, (const float*)c->lvl[c->cur][q],
+                               (const lo_t*)c->lvlo[c->cur][q], c->plane, c->pz, ex, ey, ez, dev + off, 0, 0.0);
+        HIPCHK(hipGetLastError());
+        off += n;
+    }
+    // the copy waits for the packing only; later steps on the context's
+    // stream run while it is in flight (they do not touch the staging slot)
+    HIPCHK(hipEventRecord(c->snap_packed[slot], c->stream));
+    HIPCHK(hipStreamWaitEvent(c->snap_stream, c->snap_packed[slot], 0));
+    HIPCHK(hipMemcpyAsync(c->snap_host + (size_t)slot * (size_t)c->snap_frame, dev,
+                          (size_t)c->snap_frame * sizeof(double), hipMemcpyDeviceToHost, c->snap_stream));
+    HIPCHK(hipEventRecord(c->snap_copied[slot], c->snap_stream));
+    c->snap_state[slot] = 1;
+    c->snap_step[slot] = c->step;
+    return PF_OK;
+}
+
+extern "C" int pf_snapshot_wait(pf_ctx* c, int32_t slot, double** host, int64_t* steps_done) {
+    if (!c || !host || !steps_done) return fail(PF_ERR_ARG, "NULL argument");
+    if (slot < 0 || slot >= c->snap_slots) return fail(PF_ERR_ARG, "no snapshot slot %d", slot);
+    if (c->snap_state[slot] == 0) return fail(PF_ERR_STATE, "snapshot slot %d holds no snapshot", slot);
+    if (c->snap_state[slot] == 1) {
+        HIPCHK(hipEventSynchronize(c->snap_copied[slot]));
+        c->snap_state[slot] = 2;
+    }
+    *host = c->snap_host + (size_t)slot * (size_t)c->snap_frame;
+    *steps_done = c->snap_step[slot];
+    return PF_OK;
+}
+
+extern "C" int pf_snapshot_release(pf_ctx* c, int32_t slot) {
+    if (!c) return fail(PF_ERR_ARG, "ctx is NULL");
+    if (slot < 0 || slot >= c->snap_slots) return fail(PF_ERR_ARG, "no snapshot slot %d", slot);
+    if (c->snap_state[slot] != 2) return fail(PF_ERR_STATE, "snapshot slot %d is not held (wait first)", slot);
+    c->snap_state[slot] = 0;
     return PF_OK;
 }
 

@@ -7,6 +7,8 @@
 
 // Material rows the tiled sweep stages in LDS.
 #define PF_TILED_MAX_MATERIALS 16
+// Snapshot slots a context can hold.
+#define PF_SNAP_MAX_SLOTS 16
 
 // One mesh on one device.  Time levels of the displacement components are
 // kept in level buffers: `cur` indexes the current level (n) and `old` the
@@ -55,6 +57,18 @@ struct pf_ctx {
     size_t scratch_bytes = 0;
     pf_batch* batch = nullptr;  // set while a batch owns the stepping
     hipStream_t batch_stream = nullptr;
+    // asynchronous snapshots: per slot a device staging frame and a pinned
+    // host frame (ux, uy, uz in the reference's shapes, FP64), the copy on
+    // its own stream
+    int snap_slots = 0;
+    long long snap_frame = 0;  // doubles per frame
+    double* snap_dev = nullptr;
+    double* snap_host = nullptr;
+    hipStream_t snap_stream = nullptr;
+    hipEvent_t snap_packed[PF_SNAP_MAX_SLOTS] = {};
+    hipEvent_t snap_copied[PF_SNAP_MAX_SLOTS] = {};
+    int snap_state[PF_SNAP_MAX_SLOTS] = {};      // 0 free, 1 in flight, 2 held by the caller
+    long long snap_step[PF_SNAP_MAX_SLOTS] = {};
 };
 
 namespace pf {

@@ -82,6 +82,10 @@ def load(path=None):
         "pf_steps_done": ([V, P(I64)], ctypes.c_int),
         "pf_get_field": ([V, I32, V, I64], ctypes.c_int),
         "pf_get_ids": ([V, V, I64], ctypes.c_int),
+        "pf_snapshot_open": ([V, I32], ctypes.c_int),
+        "pf_snapshot": ([V, I32], ctypes.c_int),
+        "pf_snapshot_wait": ([V, I32, P(P(D)), P(I64)], ctypes.c_int),
+        "pf_snapshot_release": ([V, I32], ctypes.c_int),
         "pf_set_field": ([V, I32, V, I64], ctypes.c_int),
         "pf_get_probes": ([V, V, I64, P(I64)], ctypes.c_int),
         "pf_energy": ([V, P(D)], ctypes.c_int),
@@ -254,6 +258,31 @@ class Context:
         return v.value
 
     # -- fields
+    # -- asynchronous snapshots (pf_snapshot_*)
+    def snapshot_open(self, slots):
+        check(self.lib.pf_snapshot_open(self.handle, int(slots)))
+
+    def snapshot(self, slot):
+        """Queue a snapshot of the current level into ``slot``; returns at once."""
+        check(self.lib.pf_snapshot(self.handle, int(slot)))
+
+    def snapshot_wait(self, slot):
+        """Wait for ``slot``'s copy; returns ({'ux','uy','uz'}: views of the
+        pinned frame, valid until snapshot_release), steps done when taken)."""
+        ptr = ctypes.POINTER(ctypes.c_double)()
+        steps = ctypes.c_int64()
+        check(self.lib.pf_snapshot_wait(self.handle, int(slot), ctypes.byref(ptr), ctypes.byref(steps)))
+        out, off = {}, 0
+        for c in ("ux", "uy", "uz"):
+            shape = self._shapes[c]
+            n = int(np.prod(shape))
+            out[c] = np.ctypeslib.as_array(ptr, shape=(off + n,))[off:].reshape(shape)
+            off += n
+        return out, steps.value
+
+    def snapshot_release(self, slot):
+        check(self.lib.pf_snapshot_release(self.handle, int(slot)))
+
     def ids(self):
         """The device's material index map, (nx, ny, nz) uint8."""
         out = np.empty(self.spec.nodes, dtype=np.uint8)

@@ -176,7 +176,8 @@ class BaseSolver:
         if self.writer.is_alive():
             self.writer.notify_finished()
             self.writer.join(timeout=1)
-        if self.cfg['write_mode'] != 'off':
+        # 'npy' (extension): the solver records into a directory sink itself
+        if self.cfg['write_mode'] not in ('off', 'npy'):
             self.writer = Writer(self.cfg['write_mode'])
             self.writer.init(steps=self.t, file=self.file, grid=self.g, cfg=self.cfg, material=self.m)
             self.writer.start()

@@ -15,13 +15,18 @@ cfg keys added to the reference's {wave, wave_args, write_mode}:
   probe         None | [i0, si, ni, j0, sj, nj] | {'stride': s} | {'count': n}
   record_every  probe frame every k steps (frames land in ``self.probes``)
   sponge        None | {'width': cells, 'strength': s}  (extension absorber)
+  record_slots  snapshot slots of the per-step recording (write_mode != 'off')
+write_mode adds 'npy' to the reference's 'off' / 'thread' / 'process': the
+Writer's datasets and attributes as a .npy directory at ``file + '.d'``
+(no h5py needed).  Every recorded frame is a device snapshot copied to the
+host while the next steps run (simulation.recording).
 """
 import logging
 from time import time
 
 import numpy as np
 
-from .. import base_solver
+from .. import base_solver, recording
 from ...gui.worker import WorkerSignals
 from ... import fdtd, problem
 
@@ -58,12 +63,26 @@ class Solver(base_solver.BaseSolver):
         self.ctx = fdtd.Context(spec, precision=self.cfg["precision"], kernel=self.cfg["kernel"])
         self.probes = None
 
+    def _recorder(self):
+        """Per-step frames from device snapshots into the Writer (HDF,
+        'thread'/'process') or a .npy directory ('npy', at self.file + '.d')."""
+        if self.cfg["write_mode"] == "npy":
+            nodes = (self.g.x.size, self.g.y.size, self.g.z.size)
+            big = int(np.prod(nodes)) > 2_000_000  # dense 6x6 stiffness: 288 B per node
+            sink = recording.NpySink(self.file + ".d", nodes, self.t,
+                                     recording.schema_attrs(self.g, self.m, self.t, self.cfg),
+                                     density=self.m.P, elasticity=None if big else self.m.C)
+        else:
+            sink = recording.WriterSink(self.writer)
+        return recording.AsyncRecorder(self.ctx, sink, slots=int(self.cfg.get("record_slots", 4)))
+
     def run(self, *args, **kwargs):
         signals = kwargs.get("signals") or WorkerSignals()
         if self.ctx is None:
             raise RuntimeError("Solver.run() before init()")
         writing = self.cfg["write_mode"] != "off"
         chunk = 1 if writing else max(1, int(self.cfg.get("chunk", 50)))
+        rec = self._recorder() if writing else None
         signals.status.emit("Solver starting..")
         self.running.set()
         signals.status.emit("Running simulation.")
@@ -77,19 +96,21 @@ class Solver(base_solver.BaseSolver):
                 break
             n = min(chunk, self.t - done)
             self.ctx.step(n)
-            self.ctx.sync()
             done += n
             if writing:
-                self._pull_fields()
-                self.writer.put((self.g.freezeData(), done - 1))
+                rec.record()  # frame done-1, copied while the next steps run
+            else:
+                self.ctx.sync()
             if progress < 99:
                 progress = min(99, int((done / self.t) * 100))
                 signals.progress.emit(progress)
+        if writing:
+            rec.close()
         self.ctx.sync()
         self.running.clear()
         self._pull_fields(old=True)
         self.probes = self.ctx.probes()
-        if writing:
+        if writing and self.cfg["write_mode"] != "npy":
             self.writer.notify_finished()
             self.writer.join(timeout=5 * 60)
         self.elapsed = time() - stime
