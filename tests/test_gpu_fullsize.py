@@ -1,0 +1,58 @@
+"""Checks at the benchmark's full sizes through properties that do not
+depend on size (the oracle cannot run these meshes in test time):
+
+  * doubling the source doubles every field and probe frame, in FP32,
+    FP32C and the two-step sweep, on the 512x512x256-cell crystal.  Every
+    operation of the step commutes with a power-of-two scaling except where
+    values are subnormal (at the numerical wavefront, where rounding has an
+    absolute quantum), so the bar is 1e-7 of each field's largest value: an
+    indexing or formula error moves the fields by O(1) of it;
+  * a homogeneous 256x256x128-cell box with Mur faces, driven by the
+    y-uniform wall line, stays mirror-symmetric in y (ux, uz even, uy odd),
+    in FP32 to the rounding of its non-mirrored operation order.  (The
+    absorbing layer is left out: it damps the staggered uy with the node
+    profile, which is not mirror-symmetric.)
+"""
+import copy
+
+import numpy as np
+import pytest
+
+from helpers import spec_from, uniform_crystal
+from phonomena_amd import fdtd, sweep
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision,kernel", [("fp32", "auto"), ("fp32c", "auto"), ("fp32", "tiled2")])
+def test_c3_doubled_source_doubles_fields(precision, kernel):
+    steps = 40
+    spec, _, _ = sweep.crystal_spec((512, 512, 256), 24, 8, 3, steps, sponge={"width": 16},
+                                    probe={"count": 64}, every=10, device_ids=True)
+    twice = copy.copy(spec)
+    twice.wave = np.ascontiguousarray(2.0 * spec.wave)
+    out = []
+    for sp in (spec, twice):
+        ctx = fdtd.Context(sp, precision=precision, kernel=kernel)
+        ctx.step(steps)
+        out.append({f: ctx.get(f) for f in ("ux", "uy", "uz", "uz_old")})
+        out[-1]["probes"] = ctx.probes()
+        ctx.close()
+    for f in out[0]:
+        scale = float(np.abs(out[1][f]).max())
+        assert scale > 0, f
+        assert float(np.abs(2.0 * out[0][f] - out[1][f]).max()) <= 1e-7 * scale, f
+
+
+def test_c2_homogeneous_box_is_mirror_symmetric_in_y():
+    steps = 300
+    x, y, z, ids, rows, dt, wave = uniform_crystal((256, 256, 128), 1.0, steps=steps, wave_kind="sin", seed=0)
+    spec = spec_from(x, y, z, ids, rows, dt, wave)
+    ctx = fdtd.Context(spec, precision="fp32")
+    ctx.step(steps)
+    for c, sign in (("ux", 1.0), ("uz", 1.0), ("uy", -1.0)):
+        u = ctx.get(c)
+        scale = float(np.abs(u).max())
+        assert scale > 0, c
+        assert float(np.abs(u - sign * u[:, ::-1, :]).max()) <= 1e-5 * scale, c
+    ctx.close()
