@@ -57,20 +57,20 @@ def crystal_spec(cells, a, r, seed, steps, probe=None, sponge=None):
     return sweep.crystal_spec(cells, a, r, seed, steps, probe=probe, sponge=sponge)
 
 
-def cpu_baseline(budget_s=20.0):
+def cpu_baseline(budget_s=15.0):
     """The oracle (NumPy restatement of the reference step, its dense C/P
     layout and expression structure) on a bounded sub-block of the same
     crystal recipe; 1 core (NumPy ufuncs are single-threaded)."""
     from oracle import reference_math as rm
     cells = (128, 128, 64)
-    spec, g, m = crystal_spec(cells, 24, 8, 3, 64, sponge=SPONGE)
+    spec, g, m = crystal_spec(cells, 24, 8, 3, 160, sponge=SPONGE)
     C, P = m.C, m.P
     pb = rm.Problem(g.x, g.y, g.z, C, P, m.dt, spec.wave, sponge=spec.sponge)
     s = rm.State(*pb.shape)
     rm.step(s, pb, 0)  # warm
     n = 0
     t0 = time.perf_counter()
-    while n < 60 and (time.perf_counter() - t0) < budget_s:
+    while n < 150 and (time.perf_counter() - t0) < budget_s:
         rm.step(s, pb, 1 + n)
         n += 1
     dt = time.perf_counter() - t0
@@ -80,6 +80,31 @@ def cpu_baseline(budget_s=20.0):
                       "16-cell absorbing layer) on a %dx%dx%d-cell block of the same crystal recipe, "
                       "%d steps, %.1f s; host has "
                       "%d CPUs (%d usable)" % (cells + (n, dt, os.cpu_count(), len(os.sched_getaffinity(0))))}
+
+
+def cpu_baseline_omp(budget_s=15.0):
+    """The C restatement of the oracle (oracle/brute.c, the reference's step
+    as per-cell loops, same bits as the NumPy one) built with OpenMP, on all
+    the host cores this process may use: a stronger CPU baseline than the
+    reference's own single-threaded NumPy cost model.  Sample: the C2-size
+    crystal recipe (256x256x128 cells), as many steps as fit the budget."""
+    from oracle import brute
+    cells = (256, 256, 128)
+    spec, g, m = crystal_spec(cells, 16, 5, 2, 400, sponge=SPONGE)
+    threads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+    state = None
+    run = lambda n, st, t0: brute.run(g.x, g.y, g.z, m.ids, spec.materials, spec.dt, spec.mur, spec.wave, n,
+                                      state=st, t0=t0, sponge=spec.sponge, omp=True)[0]
+    state = run(1, None, 0)  # warm
+    n, t = 0, time.perf_counter()
+    while n < 399 and time.perf_counter() - t < budget_s:
+        state = run(1, state, 1 + n)
+        n += 1
+    dt = time.perf_counter() - t
+    return {"value": spec.cells * n / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": "oracle/brute.c (per-cell C restatement of base_solver.py's step, bit-identical to the "
+                      "NumPy oracle) built with OpenMP, %d threads, on the %dx%dx%d-cell crystal recipe "
+                      "(a=16 r=5 seed 2, 16-cell absorbing layer), %d steps, %.1f s" % ((threads,) + cells + (n, dt))}
 
 
 def stream_peaks(nodes):
@@ -332,6 +357,8 @@ def main():
     if not args.no_setup:
         rec["setup"] = bench_setup(args.workload)
     rec["cpu_baseline"] = None if args.no_cpu else cpu_baseline()
+    if not args.no_cpu:
+        rec["cpu_baseline_omp"] = cpu_baseline_omp()
     print(json.dumps(rec), flush=True)
 
 

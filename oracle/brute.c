@@ -26,6 +26,15 @@
 
 #define I3(a, b, c, NB, NC) ((((size_t)(a)) * (NB) + (b)) * (NC) + (c))
 
+/* Built with -fopenmp (liboracle_brute_omp.so) the volume loops split over
+   x planes; every cell is computed exactly as in the serial build (no
+   reductions), so both give the same bits. */
+#ifdef _OPENMP
+#define OMP_FOR _Pragma("omp parallel for schedule(static)")
+#else
+#define OMP_FOR
+#endif
+
 typedef struct {
     int nx, ny, nz;
     const double *fdx, *fdy, *fdz, *sdx, *sdy, *sdz;
@@ -77,6 +86,7 @@ static void step(const prob_t *p, double w,
     for (int j = 0; j < ny; j++) UZ(0, j, 0) = w;
 
     /* stage 1: interior stresses */
+    OMP_FOR
     for (int i = 1; i <= nx - 2; i++)
         for (int j = 1; j <= ny - 2; j++)
             for (int k = 1; k <= nz - 2; k++) {
@@ -91,18 +101,21 @@ static void step(const prob_t *p, double w,
                     TN(Ts[a], i, j, k) = t;
                 }
             }
+    OMP_FOR
     for (int i = 1; i <= nx - 2; i++)
         for (int j = 0; j <= ny - 2; j++)
             for (int k = 0; k <= nz - 2; k++)
                 TT4(i, j, k) = cm(p, i, j + 1, k + 1, 9) *
                                ((UY(i, j, k + 1) - UY(i, j, k)) / p->fdz[k] +
                                 (UZ(i, j + 1, k) - UZ(i, j, k)) / p->fdy[j]);
+    OMP_FOR
     for (int i = 0; i <= nx - 2; i++)
         for (int j = 1; j <= ny - 2; j++)
             for (int k = 0; k <= nz - 2; k++)
                 TT5(i, j, k) = cm(p, i + 1, j, k + 1, 10) *
                                ((UX(i, j, k + 1) - UX(i, j, k)) / p->fdz[k] +
                                 (UZ(i + 1, j, k) - UZ(i, j, k)) / p->fdx[i]);
+    OMP_FOR
     for (int i = 0; i <= nx - 2; i++)
         for (int j = 0; j <= ny - 2; j++)
             for (int k = 1; k <= nz - 2; k++)
@@ -142,6 +155,7 @@ static void step(const prob_t *p, double w,
                             (UY(i + 1, j, 0) - UY(i, j, 0)) / p->fdz[0]);
 
     /* stage 3: interior displacements */
+    OMP_FOR
     for (int i = 0; i <= nx - 2; i++)
         for (int j = 1; j <= ny - 2; j++)
             for (int k = 1; k <= nz - 2; k++) {
@@ -151,6 +165,7 @@ static void step(const prob_t *p, double w,
                 UXN(i, j, k) = leap(UX(i, j, k), UXO(i, j, k), p->D / rho(p, i + 1, j, k), div,
                                     dmp ? damp(p, i, j, k) : 0, dmp);
             }
+    OMP_FOR
     for (int i = 1; i <= nx - 2; i++)
         for (int j = 0; j <= ny - 2; j++)
             for (int k = 1; k <= nz - 2; k++) {
@@ -160,6 +175,7 @@ static void step(const prob_t *p, double w,
                 UYN(i, j, k) = leap(UY(i, j, k), UYO(i, j, k), p->D / rho(p, i, j + 1, k), div,
                                     dmp ? damp(p, i, j, k) : 0, dmp);
             }
+    OMP_FOR
     for (int i = 1; i <= nx - 2; i++)
         for (int j = 1; j <= ny - 2; j++)
             for (int k = 0; k <= nz - 2; k++) {

@@ -8,23 +8,24 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = None
+_LIBS = {}
 
 
-def _lib():
-    global _LIB
-    if _LIB is None:
-        path = os.path.join(_HERE, "liboracle_brute.so")
+def _lib(omp=False):
+    """The serial library, or the OpenMP build (same bits, all cores)."""
+    if omp not in _LIBS:
+        name = "liboracle_brute_omp.so" if omp else "liboracle_brute.so"
+        path = os.path.join(_HERE, name)
         if not os.path.exists(path):
-            raise RuntimeError("oracle/liboracle_brute.so missing: run `make -C oracle`")
+            raise RuntimeError("oracle/%s missing: run `make -C oracle`" % name)
         lib = ctypes.CDLL(path)
         P = ctypes.c_void_p
         lib.oracle_run.restype = ctypes.c_int
         lib.oracle_run.argtypes = ([ctypes.c_int] * 3 + [P] * 6 + [P, P, ctypes.c_double, P]
                                    + [P] * 3 + [P, ctypes.c_int, ctypes.c_int] + [P] * 9
                                    + [P, ctypes.c_int, P])
-        _LIB = lib
-    return _LIB
+        _LIBS[omp] = lib
+    return _LIBS[omp]
 
 
 def _p(a):
@@ -42,7 +43,7 @@ def material_table(c_list, p_list):
 
 
 def run(x, y, z, ids, table, dt, mur12, wave, steps, state=None, t0=0,
-        probe=None, every=0, sponge=None, si=1.0):
+        probe=None, every=0, sponge=None, si=1.0, omp=False):
     """Run the scalar oracle; returns (state dict, frames or None).
 
     ``state`` maps ux, uy, uz, ux_new, ..., uz_old to float64 arrays of the
@@ -76,7 +77,7 @@ def run(x, y, z, ids, table, dt, mur12, wave, steps, state=None, t0=0,
     if sponge is not None:
         dxs, dys, dzs = (np.ascontiguousarray(a, dtype=np.float64) for a in sponge)
     D = dt ** 2
-    n = _lib().oracle_run(nx, ny, nz, _p(fdx), _p(fdy), _p(fdz), _p(sdx), _p(sdy), _p(sdz),
+    n = _lib(omp).oracle_run(nx, ny, nz, _p(fdx), _p(fdy), _p(fdz), _p(sdx), _p(sdy), _p(sdz),
                           _p(ids), _p(table), D, _p(mur12), _p(dxs), _p(dys), _p(dzs),
                           _p(wave), t0, steps,
                           *[_p(state["u" + c + suf]) for suf in ("", "_new", "_old") for c in "xyz"],

@@ -30,15 +30,17 @@ def test_numpy_oracle_matches_reference_trajectory(name):
                 assert np.array_equal(getattr(s2, "u" + c), d["full_u" + c][n])
 
 
+@pytest.mark.parametrize("omp", [False, True])
 @pytest.mark.parametrize("name", CASES)
-def test_scalar_oracle_matches_reference_trajectory(name):
+def test_scalar_oracle_matches_reference_trajectory(name, omp):
+    # the serial and the OpenMP build of the C restatement, bit for bit
     d = golden(name)
     pb = oracle_problem(d)
     mc = pb.mur_c
     mur12 = np.array(list(mc["xmax"]) + list(mc["y0"]) + list(mc["ymax"]) + list(mc["zmax"]))
     x, y = d["x"], d["y"]
     st, frames = brute.run(x, y, d["z"], d["ids"], rows_of(d), pb.dt, mur12, pb.wave, int(d["steps"]),
-                           probe=(0, 1, x.size, 0, 1, y.size), every=record_every(name))
+                           probe=(0, 1, x.size, 0, 1, y.size), every=record_every(name), omp=omp)
     for f in FIELDS:
         key = f.replace("_old", "_old")
         assert np.array_equal(st[key], d[f]), f
