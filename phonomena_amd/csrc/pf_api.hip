@@ -469,6 +469,9 @@ static int launch_faces_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
 template <class R>
 static int launch_probe_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
     if (c->probe[2] <= 0) return PF_OK;
+    // a step index known on the host (no device counter): launch only on
+    // the steps that record a frame (the kernel checks it again)
+    if (!m.step_base && (m.step_off >= m.nsteps || (m.step_off + 1) % c->every != 0)) return PF_OK;
     // newest level after this step = the n* buffers of the view
     hipLaunchKernelGGL((k_probe<R>), dim3(1), dim3(256), 0, s, (const R*)m.n2, (const lo_t*)m.nl2, 0,
                        c->plane, c->pz, c->probe[0], c->probe[1], c->probe[2], c->probe[3], c->probe[4],
