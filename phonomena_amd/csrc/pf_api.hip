@@ -223,6 +223,8 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
         return fail(PF_ERR_ARG, "the tiled and strip sweeps hold at most %d materials (got %d)",
                     PF_TILED_MAX_MATERIALS, d->n_materials);
     c->tile_x = d->tile_x > 0 ? d->tile_x : 0;
+    if (c->kernel == PF_KERNEL_TILED && !small)
+        return fail(PF_ERR_ARG, "the tiled sweep needs a time level < 4 GiB (use the naive kernel)");
     const bool pair = c->kernel == PF_KERNEL_TILED2;
     if (pair) {
         if (d->precision == PF_FP64)
