@@ -157,3 +157,26 @@ def test_batch_with_sponge_and_rules():
     b.close()
     for x in ctxs + [plain, other]:
         x.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp32c"])
+def test_two_step_context_through_a_batch(precision):
+    # a two-step context (four level buffers) steps in pairs, joins a batch
+    # (which drives its current and previous level), leaves it and steps in
+    # pairs again: the same bits as stepping alone with the one-step sweep
+    spec = _spec(*MIX[0])
+    ref = _alone(spec, precision, 90)
+    ctx = fdtd.Context(spec, precision=precision, kernel="tiled2")
+    ctx.step(7)  # three pairs and one single step
+    b = fdtd.Batch(precision)
+    b.add(ctx)
+    b.step(21)
+    b.sync()
+    b.remove(ctx)
+    b.close()
+    ctx.step(62)
+    ctx.sync()
+    out = _state(ctx)
+    ctx.close()
+    for f in ref:
+        assert np.array_equal(out[f], ref[f]), f
