@@ -286,6 +286,11 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
     c->dt = d->dt;
     c->nmat = d->n_materials;
     HIPCHK(hipGetDevice(&c->dev));
+    // sizes the x chunks of the tiled sweeps (0 keeps the fixed rule)
+    if (hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, c->dev) != hipSuccess) {
+        (void)hipGetLastError();
+        c->cus = 0;
+    }
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
 
     // fields: two levels (four for the two-step sweep) x three components
@@ -502,22 +507,24 @@ static int step_once_t(pf_ctx* c, hipStream_t s, const long long* base, long lon
     return PF_OK;
 }
 
+template <class R, int MODE, bool SPONGE, int TJ, int KZ>
+static int launch_pair_k(const pf_ctx* c, const Mesh<R>& m, R* out2, hipStream_t s) {
+    static const int per_cu = blocks_per_cu(k_sweep_fused<R, MODE, SPONGE, TJ, KZ>, TJ * KZ);
+    const TileMap tm = fused_map(c, TJ, KZ, c->tile_x, (long long)c->cus * per_cu);
+    const long long nb = (long long)tm.nkt * tm.njt * tm.nxc;
+    if (nb > 0x7fffffffLL) return fail(PF_ERR_ARG, "too many tiles");
+    hipLaunchKernelGGL((k_sweep_fused<R, MODE, SPONGE, TJ, KZ>), dim3((unsigned)nb), dim3(64, TJ * KZ / 64), 0, s, m,
+                       out2, tm);
+    HIPCHK(hipGetLastError());
+    return PF_OK;
+}
+
 template <class R, int MODE, bool SPONGE>
 static int launch_pair_s(const pf_ctx* c, const Mesh<R>& m, R* out2, hipStream_t s) {
     const int tj = c->pair_j, kz = c->pair_z;
-    const TileMap tm = fused_map(c, tj, kz, c->tile_x);
-    const long long nb = (long long)tm.nkt * tm.njt * tm.nxc;
-    if (nb > 0x7fffffffLL) return fail(PF_ERR_ARG, "too many tiles");
-    if (kz == 64 && tj == 16)
-        hipLaunchKernelGGL((k_sweep_fused<R, MODE, SPONGE, 16, 64>), dim3((unsigned)nb), dim3(64, 16), 0, s, m, out2,
-                           tm);
-    else if (kz == 32 && tj == 32)
-        hipLaunchKernelGGL((k_sweep_fused<R, MODE, SPONGE, 32, 32>), dim3((unsigned)nb), dim3(64, 16), 0, s, m, out2,
-                           tm);
-    else
-        return fail(PF_ERR_ARG, "two-step sweep: tile_j %d with tile_z %d is not a built shape", tj, kz);
-    HIPCHK(hipGetLastError());
-    return PF_OK;
+    if (kz == 64 && tj == 16) return launch_pair_k<R, MODE, SPONGE, 16, 64>(c, m, out2, s);
+    if (kz == 32 && tj == 32) return launch_pair_k<R, MODE, SPONGE, 32, 32>(c, m, out2, s);
+    return fail(PF_ERR_ARG, "two-step sweep: tile_j %d with tile_z %d is not a built shape", tj, kz);
 }
 
 // Steps tt, tt+1: the two-step pass writes level n+1 (view m1: n, n-1 ->
@@ -542,11 +549,13 @@ static int step_pair_t(pf_ctx* c, hipStream_t s, const long long* base, long lon
     if (ev) HIPCHK(hipEventRecord(ev[1], s));
     if ((rc = launch_faces_t<R, MODE>(c, m1, s))) return rc;
     if ((rc = launch_probe_t<R>(c, m1, s))) return rc;
-    const long long nb = band_total(c->nx, c->ny, c->nz);
+    const BandMap bm = band_map(c);
+    const long long nb = band_blocks(bm);
+    if (nb > 0x7fffffffLL) return fail(PF_ERR_ARG, "too many tiles");
     if (c->sponge)
-        hipLaunchKernelGGL((k_band<R, MODE, true>), dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, m2, nb);
+        hipLaunchKernelGGL((k_band<R, MODE, true>), dim3((unsigned)nb), dim3(64, 4), 0, s, m2, bm.j, bm.i);
     else
-        hipLaunchKernelGGL((k_band<R, MODE, false>), dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, m2, nb);
+        hipLaunchKernelGGL((k_band<R, MODE, false>), dim3((unsigned)nb), dim3(64, 4), 0, s, m2, bm.j, bm.i);
     HIPCHK(hipGetLastError());
     if ((rc = launch_faces_t<R, MODE>(c, m2, s))) return rc;
     if ((rc = launch_probe_t<R>(c, m2, s))) return rc;

@@ -22,7 +22,7 @@
 // from the lane below and covers the top of the columns.  The x and y faces
 // are final only after the face kernel, so stage B stores no cell whose
 // stencil reads one of them: it stores i <= nx-4, 2 <= j <= ny-4.  The
-// one-step per-cell code updates the remaining band (k_band) after the
+// one-step tiled march updates the remaining band (k_band) after the
 // level n+1 faces, then the level n+2 faces are written.
 //
 // Stage B sees level n+1 exactly as a one-step sweep loads it: 0 outside
@@ -425,12 +425,16 @@ PF_DEV void fused_march(const Mesh<R>& m, R* out2, FusedLds<R, TJ, KZ>& L, int k
     }
 }
 
-// Tiles of the two-step sweep: stride KZ-4 lanes by TJ-4 rows.
-inline TileMap fused_map(const pf_ctx* c, int tj, int kz, int xchunk) {
+// Tiles of the two-step sweep: stride KZ-4 lanes by TJ-4 rows.  x chunks
+// as tile_map chooses them; a block marches three planes beyond its chunk
+// (two warm-up planes, one trailing plane of stage B, rounded to a pair).
+inline TileMap fused_map(const pf_ctx* c, int tj, int kz, int xchunk, long long slots = 0) {
     TileMap tm;
     tm.nkt = (c->nz + (kz - 4) - 1) / (kz - 4);
     tm.njt = (c->ny + (tj - 4) - 1) / (tj - 4);
-    if (xchunk <= 0) {
+    if (xchunk <= 0 && slots > 0) {
+        xchunk = round_xchunk((long long)tm.nkt * tm.njt, c->nx, 3, 32, XT2 - 6, slots);
+    } else if (xchunk <= 0) {
         // aim for ~1024 blocks (4 per CU) with chunks of at least 32 planes
         const long long per = (long long)tm.nkt * tm.njt;
         long long want = (1024 + per - 1) / per;
@@ -489,43 +493,44 @@ __global__ __launch_bounds__(TJ * KZ, 1) void k_sweep_fused(Mesh<R> m, R* out2, 
 }
 
 // ----------------------------------------------------------- band ----
-// Cells the two-step pass leaves to the one-step code, z fastest:
-//   J: j in {1, ny-3, ny-2}, k <= nz-2, i <= nx-2
+// Cells the two-step pass leaves to the one-step sweep:
+//   J: j in {1, ny-3, ny-2}, i <= nx-2, k <= nz-2
 //   I: i in {nx-3, nx-2}, 2 <= j <= ny-4, k <= nz-2
 // (no component has a non-face cell at i = nx-1, j = 0, j = ny-1 or
 // k = nz-1; the z = max faces above k = nz-2 / nz-3 come with their cell).
-// Needs nx, ny, nz >= 8.
-inline long long band_total(int nx, int ny, int nz) {
-    return 3LL * (nx - 1) * (nz - 1) + 2LL * (ny - 5) * (nz - 1);
+// The one-step tiled march sweeps them on tiles of 4 rows x 64 lanes:
+//   J tiles: two tile rows, rows -1..2 (writer rows 0 and 1; row 0 has no
+//            non-face cell) and rows ny-4..ny-1 (writer rows ny-3, ny-2),
+//            over all planes in chunks of 8;
+//   I tiles: every tile row (stride 2), over the one chunk of the planes
+//            nx-3, nx-2.
+// So every tile stores band cells only; the J cells of planes nx-3, nx-2
+// are stored by a J and an I tile, from the same inputs with the same
+// code, so with the same bits.  Needs nx, ny, nz >= 8.
+struct BandMap {
+    TileMap j, i;
+};
+
+inline BandMap band_map(const pf_ctx* c) {
+    BandMap b;
+    b.j = tile_map(c, 4, 64, 8);
+    b.j.njt = 2;
+    b.i = tile_map(c, 4, 64, 2);
+    b.i.nxc = 1;
+    return b;
 }
 
-PF_DEV void band_cell(int nx, int ny, int nz, long long t, int& i, int& j, int& k) {
-    const int kz = nz - 1;
-    const long long n = 3LL * (nx - 1) * kz;
-    if (t < n) {
-        k = (int)(t % kz);
-        const long long r = t / kz;
-        const int s = (int)(r % 3);
-        j = s == 0 ? 1 : (s == 1 ? ny - 3 : ny - 2);
-        i = (int)(r / 3);
-        return;
-    }
-    t -= n;
-    k = (int)(t % kz);
-    const long long r = t / kz;
-    j = 2 + (int)(r % (ny - 5));
-    i = nx - 3 + (int)(r / (ny - 5));
+inline long long band_blocks(const BandMap& b) {
+    return (long long)b.j.nkt * b.j.njt * b.j.nxc + (long long)b.i.nkt * b.i.njt * b.i.nxc;
 }
 
 template <class R, int MODE, bool SPONGE>
-__global__ __launch_bounds__(256) void k_band(Mesh<R> m, long long total) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= total) return;
-    const long long tt = step_index(m);
-    if (tt >= m.nsteps) return;
-    int i, j, k;
-    band_cell(m.nx, m.ny, m.nz, t, i, j, k);
-    naive_cell<R, MODE, SPONGE>(m, i, j, k, (R)m.wave[tt]);
+__global__ __launch_bounds__(256, (MODE == MODE_FP32 ? 4 : 2)) void k_band(Mesh<R> m, TileMap tj, TileMap ti) {
+    __shared__ TiledLds<R, 4, 64> L;
+    const int nj = tj.nkt * tj.njt * tj.nxc;
+    const int b = (int)blockIdx.x;
+    const bool jb = b < nj;
+    tiled_tile<R, MODE, SPONGE, 4, 64>(m, jb ? tj : ti, jb ? b : b - nj, L, jb ? m.ny - 3 : 2, jb ? 0 : m.nx - 3);
 }
 
 }  // namespace pf

@@ -27,6 +27,7 @@ struct pf_ctx {
     double dt = 0;
     int nmat = 0;
     int dev = 0;
+    int cus = 0;  // compute units of the device (0: unknown)
     hipStream_t stream = nullptr;
     void* lvlbase[4] = {};
     void* lvl[4][3] = {};

@@ -487,10 +487,13 @@ PF_DEV void tiled_march(const Mesh<R>& m, TiledLds<R, TJ, KZ>& L, int kl, int ro
 }
 
 // One tile of a sweep: tile index t (after any remapping) of mesh m under
-// tile map tm, LDS L.  Every thread of the block calls it; it ends without
-// a barrier, so a caller that reuses L for another tile syncs first.
+// tile map tm, LDS L.  Tile row jt starts at row jt * jstep - 1 and x chunk
+// xc at plane x0 + xc * xchunk (a sweep of the whole mesh: jstep TJ-2, x0
+// 0).  Every thread of the block calls it; it ends without a barrier, so a
+// caller that reuses L for another tile syncs first.
 template <class R, int MODE, bool SPONGE, int TJ, int KZ>
-PF_DEV void tiled_tile(const Mesh<R>& m, const TileMap& tm, int t, TiledLds<R, TJ, KZ>& L) {
+PF_DEV void tiled_tile(const Mesh<R>& m, const TileMap& tm, int t, TiledLds<R, TJ, KZ>& L, int jstep = TJ - 2,
+                       int x0 = 0) {
     const int kl = threadIdx.x % KZ;
     const int row = threadIdx.y * (64 / KZ) + threadIdx.x / KZ;
     const int kt = t % tm.nkt;
@@ -498,8 +501,8 @@ PF_DEV void tiled_tile(const Mesh<R>& m, const TileMap& tm, int t, TiledLds<R, T
     const int jt = r_ % tm.njt;
     const int xc = r_ / tm.njt;
     const int k = kt * (KZ - 2) - 1 + kl;
-    const int j = jt * (TJ - 2) - 1 + row;
-    const int ib = xc * tm.xchunk;
+    const int j = jt * jstep - 1 + row;
+    const int ib = x0 + xc * tm.xchunk;
     const int ie = min(ib + tm.xchunk, m.nx);
     const long long tt = step_index(m);
     if (tt >= m.nsteps) return;  // uniform over the grid
@@ -518,7 +521,7 @@ PF_DEV void tiled_tile(const Mesh<R>& m, const TileMap& tm, int t, TiledLds<R, T
     }
     // block-uniform: the whole tile and chunk are clear of every boundary
     const int k_lo = kt * (KZ - 2) - 1, k_hi = k_lo + KZ - 1;
-    const int j_lo = jt * (TJ - 2) - 1, j_hi = j_lo + TJ - 1;
+    const int j_lo = jt * jstep - 1, j_hi = j_lo + TJ - 1;
     const bool full = k_lo >= 1 && k_hi <= m.nz - 4 && j_lo >= 1 && j_hi <= m.ny - 3 && ib >= 2 && ie <= m.nx - 3;
     if (full)
         tiled_march<R, MODE, SPONGE, TJ, KZ, true>(m, L, kl, row, k, j, ib, ie, w);
@@ -534,11 +537,54 @@ __global__ __launch_bounds__(TJ * KZ, (MODE == MODE_FP32 ? 4 : 2)) void k_sweep_
     tiled_tile<R, MODE, SPONGE, TJ, KZ>(m, tm, xcd_remap((int)blockIdx.x, (int)gridDim.x), L);
 }
 
-inline TileMap tile_map(const pf_ctx* c, int tj, int kz, int xchunk) {
+// Blocks of a kernel that one CU holds at once (1 when the runtime cannot
+// say).
+template <class K> int blocks_per_cu(K kernel, int threads) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, 0) != hipSuccess || n < 1) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return n;
+}
+
+// Planes per x chunk, from the rounds in which the device runs the blocks.
+// per tiles per plane, each block marching xc planes plus `warm` planes it
+// only loads and computes; `slots` blocks resident at once.  Over the
+// balanced chunkings (n chunks of ceil(nx / n) planes, lo <= planes <= cap)
+// the one with the most useful planes per resident-block plane time:
+//     per * nx / (slots * rounds * (xc + warm)),  rounds = ceil(per * n / slots)
+// which charges a last round that leaves CUs idle and the warm-up planes
+// of short chunks.  Ties go to more chunks.
+inline int round_xchunk(long long per, int nx, int warm, int lo, int cap, long long slots) {
+    int best = nx < cap ? nx : cap;
+    double best_s = -1.0;
+    for (int n = 1; n <= nx; n++) {
+        const int xc = (nx + n - 1) / n;
+        if (xc > cap) continue;
+        if (xc < lo && n > 1) break;
+        const int nc = (nx + xc - 1) / xc;
+        if (nc != n) continue;  // same chunking as a smaller n
+        const long long blocks = per * nc;
+        const long long rounds = (blocks + slots - 1) / slots;
+        const double sc = (double)per * nx / ((double)slots * (double)rounds * (double)(xc + warm));
+        if (sc >= best_s) {
+            best_s = sc;
+            best = xc;
+        }
+    }
+    return best;
+}
+
+// slots > 0: x chunks by round_xchunk for that many resident blocks;
+// otherwise a fixed rule.  An explicit xchunk > 0 is kept (within range).
+inline TileMap tile_map(const pf_ctx* c, int tj, int kz, int xchunk, long long slots = 0) {
     TileMap tm;
     tm.nkt = (c->nz + (kz - 2) - 1) / (kz - 2);
     tm.njt = (c->ny + (tj - 2) - 1) / (tj - 2);
-    if (xchunk <= 0) {
+    if (xchunk <= 0 && slots > 0) {
+        xchunk = round_xchunk((long long)tm.nkt * tm.njt, c->nx, 1, 16, XT - 3, slots);
+    } else if (xchunk <= 0) {
         // aim for >= ~2048 blocks (8 per CU) with chunks of at least 16 planes
         const long long per = (long long)tm.nkt * tm.njt;
         long long want = (2048 + per - 1) / per;
@@ -561,7 +607,8 @@ inline double tiled_slots(const pf_ctx* c, int tj, int kz) {
 
 template <class R, int MODE, bool SPONGE, int TJ, int KZ>
 int launch_tiled_k(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
-    TileMap tm = tile_map(c, TJ, KZ, c->tile_x);
+    static const int per_cu = blocks_per_cu(k_sweep_tiled<R, MODE, SPONGE, TJ, KZ>, TJ * KZ);
+    TileMap tm = tile_map(c, TJ, KZ, c->tile_x, (long long)c->cus * per_cu);
     const long long nb = (long long)tm.nkt * tm.njt * tm.nxc;
     if (nb > 0x7fffffffLL) return fail(PF_ERR_ARG, "too many tiles");
     hipLaunchKernelGGL((k_sweep_tiled<R, MODE, SPONGE, TJ, KZ>), dim3((unsigned)nb), dim3(64, TJ * KZ / 64), 0, s, m,
