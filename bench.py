@@ -43,6 +43,11 @@ BYTES_PER_NODE = {"fp32": 12 * 4, "fp32c": 18 * 4, "fp64": 12 * 8}
 # what this sweep itself must move: 9 field values (FP32C: 4-byte hi and
 # 2-byte lo each) and the 1-byte material index
 KERNEL_BYTES_PER_NODE = {"fp32": 9 * 4 + 1, "fp32c": 9 * (4 + 2) + 1, "fp64": 9 * 8 + 1}
+# the two-step pass per step: levels n, n-1 read and n+1, n+2 written once
+# per two steps (12 field values) and the index once
+PAIR_BYTES_PER_NODE = {"fp32": (12 * 4 + 1) / 2, "fp32c": (12 * (4 + 2) + 1) / 2}
+SWEEP_KERNEL = {"tiled": "one-step tiled sweep (k_sweep_tiled)", "tiled2": "two-step sweep (k_sweep_fused)",
+                "naive": "per-node sweep (k_sweep_naive)", "strip": "strip sweep (k_sweep_strip)"}
 WORKLOADS = {
     "plain": dict(cells=(512, 512, 256), a=24, r=8, seed=3),
     "c2": dict(cells=(256, 256, 128), a=16, r=5, seed=2),
@@ -115,14 +120,16 @@ def stream_peaks(nodes):
     return {"copy_1r1w": fdtd.stream_probe(1, 1, n, 10)[0], "mix_6r3w": fdtd.stream_probe(6, 3, n, 10)[0]}
 
 
-def traffic_from_profiles(workload, precision):
+def traffic_from_profiles(workload, precision, kernel):
+    """HBM bytes per launch of the sweep kernel, from the FETCH_SIZE /
+    WRITE_SIZE summaries under profiles/ (None when not measured)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     try:
         with open(path) as fh:
             t = json.load(fh)
-        return t.get("%s/%s" % (workload, precision))
+        return t.get("%s/%s/%s" % (workload, precision, kernel))
     except (OSError, ValueError):
         return None
 
@@ -136,6 +143,8 @@ def bench_plain(args):
     nodes = spec.cells
     ctx = fdtd.Context(spec, precision=args.precision, kernel=args.kernel, tile_j=args.tile_j,
                        tile_x=args.tile_x, tile_z=args.tile_z)
+    kernel, tile_j, tile_z = ctx.kernel_info()
+    per_launch = 2 if kernel == "tiled2" else 1  # steps per sweep launch
     ctx.step(args.warmup)
     ctx.sync()
     t0 = time.perf_counter()
@@ -164,13 +173,16 @@ def bench_plain(args):
                                                           (wl["a"], wl["r"], wl["seed"],
                                                            "no absorbing layer" if args.no_sponge
                                                            else "16-cell absorbing layer")),
-                   "nodes": nodes, "kernel": args.kernel, "tile_j": args.tile_j, "tile_x": args.tile_x,
-                   "tile_z": args.tile_z,
+                   "nodes": nodes, "kernel": kernel, "kernel_requested": args.kernel, "tile_j": tile_j,
+                   "tile_z": tile_z, "tile_x": args.tile_x,
                    "fill_fraction": float(m.ids.mean())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_from_profiles(args.workload, args.precision),
-                     "kernel": "sweep", "bytes_per_node": bpn, "kernel_bytes_per_node": KERNEL_BYTES_PER_NODE[args.precision],
+                     "traffic": traffic_from_profiles(args.workload, args.precision, kernel),
+                     "kernel": SWEEP_KERNEL[kernel], "steps_per_launch": per_launch,
+                     "launch_ms": sweep_ms * per_launch, "bytes_per_node": bpn,
+                     "kernel_bytes_per_node": (PAIR_BYTES_PER_NODE if kernel == "tiled2"
+                                               else KERNEL_BYTES_PER_NODE)[args.precision],
                      "sweep_ms": sweep_ms, "face_ms": face_ms},
     }
     ctx.close()

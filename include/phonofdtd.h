@@ -37,8 +37,11 @@ extern "C" {
 #define PF_FP32C 2  /* FP32 fields carried as hi+lo pairs: compensated
                        (TwoSum/Kahan) time integration for long runs         */
 
-/* Sweep kernel. */
+/* Sweep kernel.  PF_KERNEL_AUTO: PF_KERNEL_TILED2 for PF_FP32 meshes of at
+   least PF_TILED2_AUTO_NODES nodes (>= 8 per axis), else PF_KERNEL_TILED;
+   PF_KERNEL_NAIVE beyond 16 materials or a 4 GiB time level. */
 #define PF_KERNEL_AUTO 0
+#define PF_TILED2_AUTO_NODES (1LL << 24)
 #define PF_KERNEL_NAIVE 1   /* one thread per node, stresses recomputed     */
 #define PF_KERNEL_TILED 2   /* LDS-staged (y,z) tiles, register march in x  */
 #define PF_KERNEL_STRIP 3   /* barrier-free: each wavefront owns a strip of
@@ -130,6 +133,13 @@ int pf_destroy(pf_ctx *ctx);
 int pf_step(pf_ctx *ctx, int64_t nsteps);
 int pf_sync(pf_ctx *ctx);
 int pf_steps_done(pf_ctx *ctx, int64_t *out);
+/*
+ * The sweep a context runs (PF_KERNEL_AUTO resolved) and its tile: rows
+ * along y and lanes along z per tile (the two-step tile for
+ * PF_KERNEL_TILED2; 0 for the naive sweep).  No counterpart in the
+ * reference, which has one solver loop; for reports and benchmarks.
+ */
+int pf_kernel_info(pf_ctx *ctx, int32_t *kernel, int32_t *tile_j, int32_t *tile_z);
 
 /* Fields in reference shapes, converted to/from double. */
 int pf_get_field(pf_ctx *ctx, int32_t which, double *host, int64_t count);

@@ -80,6 +80,7 @@ def load(path=None):
         "pf_step": ([V, I64], ctypes.c_int),
         "pf_sync": ([V], ctypes.c_int),
         "pf_steps_done": ([V, P(I64)], ctypes.c_int),
+        "pf_kernel_info": ([V, P(I32), P(I32), P(I32)], ctypes.c_int),
         "pf_get_field": ([V, I32, V, I64], ctypes.c_int),
         "pf_get_ids": ([V, V, I64], ctypes.c_int),
         "pf_snapshot_open": ([V, I32], ctypes.c_int),
@@ -256,6 +257,14 @@ class Context:
         v = ctypes.c_int64()
         check(self.lib.pf_steps_done(self.handle, ctypes.byref(v)))
         return v.value
+
+    def kernel_info(self):
+        """(kernel name, tile rows along y, tile lanes along z) of the sweep
+        this context runs, 'auto' resolved."""
+        k, j, z = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        check(self.lib.pf_kernel_info(self.handle, ctypes.byref(k), ctypes.byref(j), ctypes.byref(z)))
+        name = {v: n for n, v in KERNEL.items()}[k.value]
+        return name, j.value, z.value
 
     # -- fields
     # -- asynchronous snapshots (pf_snapshot_*)

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from helpers import CASES, golden, spec_from, spec_from_golden, uniform_crystal
-from phonomena_amd import fdtd, problem
+from phonomena_amd import fdtd, problem, sweep
 
 pytestmark = pytest.mark.gpu
 FIELDS = ["ux", "uy", "uz", "ux_old", "uy_old", "uz_old"]
@@ -96,3 +96,25 @@ def test_pair_rejects_fp64_and_other_tiles():
         fdtd.Context(spec, precision="fp64", kernel="tiled2")
     with pytest.raises(fdtd.FdtdError, match="16 x 64 or 32 x 32"):
         fdtd.Context(spec, precision="fp32", kernel="tiled2", tile_j=8, tile_z=64)
+
+
+@pytest.mark.parametrize("cells,precision,want", [
+    ((256, 256, 256), "fp32", "tiled2"),   # 257^3 nodes >= 2^24
+    ((256, 256, 252), "fp32", "tiled"),    # 257 x 257 x 253 nodes < 2^24
+    ((256, 256, 256), "fp32c", "tiled"),
+    ((256, 256, 256), "fp64", "tiled"),
+])
+def test_auto_resolves_by_size_and_precision(cells, precision, want):
+    """'auto' runs the two-step sweep for FP32 meshes of at least 2^24 nodes
+    and the one-step sweep otherwise, as pf_kernel_info reports."""
+    spec, _, _ = sweep.crystal_spec(cells, 24, 8, 3, 4, device_ids=True)
+    ctx = fdtd.Context(spec, precision=precision)
+    try:
+        kernel, tj, tz = ctx.kernel_info()
+        assert kernel == want
+        shapes = {(16, 64), (32, 32)} if precision != "fp64" else {(8, 64), (8, 32)}
+        assert (tj, tz) in shapes
+        ctx.step(3)  # one pair and one single step
+        assert ctx.steps_done == 3
+    finally:
+        ctx.close()
