@@ -206,7 +206,7 @@ def bench_setup(workload="plain", precision="fp32"):
         g, m = lattice.crystal(wl["cells"], wl["a"], wl["r"], wl["seed"], host_ids=(mode == "host"))
         t1 = time.perf_counter()
         spec = problem.mesh_spec(g, m, np.zeros(2), {}, device_ids=(mode == "device"))
-        ctx = fdtd.Context(spec, precision=precision)
+        ctx = fdtd.Context(spec, precision=precision, tune=False)  # map set-up only, no shape timing
         ctx.sync()
         t2 = time.perf_counter()
         maps.append(ctx.ids())
@@ -230,7 +230,7 @@ def bench_batched(steps, precisions=("fp32c", "fp32"), n=64, seed=64, trial_step
     specs = [sweep.job_spec(j, probe={"count": 32}, every=10) for j in jobs]
     nodes = sum(sp.cells for sp in specs)
     # FP64 fields of the same meshes, as one FP64 batch
-    c64 = [fdtd.Context(sp, precision="fp64") for sp in specs]
+    c64 = [fdtd.Context(sp, precision="fp64", tune=False) for sp in specs]
     b64 = fdtd.Batch("fp64")
     for c in c64:
         b64.add(c)
@@ -242,15 +242,22 @@ def bench_batched(steps, precisions=("fp32c", "fp32"), n=64, seed=64, trial_step
         c.close()
     out = {}
     for prec in precisions:
-        t_plain = 0.0
+        # the plain loop as the library runs it (tile shapes measured at
+        # creation, not timed, like the upload) and with the fixed shape
+        # the library used before it measured shapes
+        t_plain, t_fixed = 0.0, 0.0
         for sp in specs:
-            c = fdtd.Context(sp, precision=prec)
-            t0 = time.perf_counter()
-            c.step(steps)
-            c.sync()
-            t_plain += time.perf_counter() - t0
-            c.close()
-        ctxs = [fdtd.Context(sp, precision=prec) for sp in specs]
+            for tune in (True, False):
+                c = fdtd.Context(sp, precision=prec, tune=tune)
+                t0 = time.perf_counter()
+                c.step(steps)
+                c.sync()
+                if tune:
+                    t_plain += time.perf_counter() - t0
+                else:
+                    t_fixed += time.perf_counter() - t0
+                c.close()
+        ctxs = [fdtd.Context(sp, precision=prec, tune=False) for sp in specs]
         b = fdtd.Batch(prec)
         for c in ctxs:
             b.add(c)
@@ -265,7 +272,9 @@ def bench_batched(steps, precisions=("fp32c", "fp32"), n=64, seed=64, trial_step
             c.close()
         out[prec] = {"value": nodes * steps / t_batch, "plain_loop": nodes * steps / t_plain,
                      "speedup_vs_plain": t_plain / t_batch, "seconds_batched": t_batch,
-                     "seconds_plain": t_plain, "rel_l2_max_vs_fp64": worst, "autotuned": tuned}
+                     "seconds_plain": t_plain, "rel_l2_max_vs_fp64": worst, "autotuned": tuned,
+                     "plain_loop_fixed_tiles": nodes * steps / t_fixed,
+                     "speedup_vs_plain_fixed_tiles": t_fixed / t_batch}
     head = out[precisions[0]]
     return {"workload": "%d-crystal mix, seed %d: 48x48x24..192x192x96 cells, lattice a 8-24 cells, r/a "
                         "0.15-0.45, void-like or tungsten-like, gauss 2 MHz source, 32x32 probes every 10 "
@@ -273,6 +282,7 @@ def bench_batched(steps, precisions=("fp32c", "fp32"), n=64, seed=64, trial_step
             "meshes": n, "nodes_total": nodes, "steps": steps, "precision": precisions[0],
             "value": head["value"], "unit": "cell-updates/s", "plain_loop": head["plain_loop"],
             "speedup_vs_plain": head["speedup_vs_plain"], "rel_l2_max_vs_fp64": head["rel_l2_max_vs_fp64"],
+            "speedup_vs_plain_fixed_tiles": head["speedup_vs_plain_fixed_tiles"],
             "by_precision": out}
 
 

@@ -147,6 +147,8 @@ static int create_tables(pf_ctx* c, const pf_desc* d) {
     return PF_OK;
 }
 
+static int tune_shape(pf_ctx* c);
+
 extern "C" int pf_create(const pf_desc* d, pf_ctx** out) {
     if (!out) return fail(PF_ERR_ARG, "out is NULL");
     *out = nullptr;
@@ -155,6 +157,7 @@ extern "C" int pf_create(const pf_desc* d, pf_ctx** out) {
     pf_ctx* c = new (std::nothrow) pf_ctx();
     if (!c) return fail(PF_ERR_NOMEM, "host allocation failed");
     rc = ctx_init(c, d);
+    if (!rc && c->shape_open) rc = tune_shape(c);
     if (rc) {
         ctx_free(c);
         delete c;
@@ -284,6 +287,7 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
         }
         c->tile_z = kz;
         c->tile_j = req_j > 0 ? req_j : (kz == 64 ? tj64 : tj32);
+        c->shape_open = c->kernel == PF_KERNEL_TILED && d->tile_j == 0 && d->tile_z == 0;
         const bool ok = kz == 64 ? (c->tile_j == 4 || c->tile_j == 8 || c->tile_j == 16)
                                  : (c->tile_j == 8 || c->tile_j == 16 || c->tile_j == 32);
         if (!ok) return fail(PF_ERR_ARG, "tile_j must be 4, 8 or 16 with tile_z 64, 8, 16 or 32 with tile_z 32 (got %d, %d)",
@@ -513,6 +517,70 @@ static int step_once_t(pf_ctx* c, hipStream_t s, const long long* base, long lon
     c->cur = c->old;
     c->old = t;
     return PF_OK;
+}
+
+// Tile shape of the one-step tiled sweep by measurement, when the caller
+// leaves it open: each built shape sweeps the new context's own fields (all
+// zero at creation, step 0) a few times on its stream, and the fastest is
+// kept.  All shapes give the same bits, so only the speed depends on the
+// choice.  Only the sweep runs (no faces, probes or step counters), and the
+// level buffers are cleared afterwards.  The best shape varies with the
+// mesh (z extent against 62- or 30-lane tile rows, blocks per CU against
+// the CU count) more than a fixed rule follows.
+template <class R, int MODE>
+static int tune_shape_t(pf_ctx* c) {
+    static const int f32[][2] = {{16, 64}, {8, 64}, {32, 32}, {16, 32}, {8, 32}};
+    static const int f64[][2] = {{8, 32}, {16, 32}, {8, 64}};
+    const bool d64 = MODE == MODE_FP64;
+    const int (*cand)[2] = d64 ? f64 : f32;
+    const int nc = d64 ? 3 : 5;
+    if (c->wave_len < 1) return PF_OK;  // the sweep would return at once
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+        if (a) (void)hipEventDestroy(a);
+        (void)hipGetLastError();
+        return fail(PF_ERR_HIP, "tile tuning: cannot create events");
+    }
+    const Mesh<R> m = make_view<R>(c, c->cur, c->old, c->old);
+    const int j0 = c->tile_j, z0 = c->tile_z;
+    int bj = j0, bz = z0, rc = PF_OK;
+    float best = 0.f;
+    for (int s = 0; s < nc && !rc; s++) {
+        c->tile_j = cand[s][0];
+        c->tile_z = cand[s][1];
+        float ms = 0.f;
+        if ((rc = launch_sweep_t<R, MODE>(c, m, c->stream))) break;  // warm-up
+        if (hipEventRecord(a, c->stream) != hipSuccess) { rc = fail(PF_ERR_HIP, "tile tuning: event"); break; }
+        for (int r = 0; r < 4 && !rc; r++) rc = launch_sweep_t<R, MODE>(c, m, c->stream);
+        if (rc) break;
+        if (hipEventRecord(b, c->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+            hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+            rc = fail(PF_ERR_HIP, "tile tuning: event timing");
+            break;
+        }
+        if (s == 0 || ms < best) {
+            best = ms;
+            bj = cand[s][0];
+            bz = cand[s][1];
+        }
+    }
+    c->tile_j = rc ? j0 : bj;
+    c->tile_z = rc ? z0 : bz;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc) return rc;
+    const size_t lb = level_bytes(c->cells, c->precision);
+    for (int l = 0; l < c->nlevels; l++) HIPCHK(hipMemsetAsync(c->lvlbase[l], 0, lb, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
+static int tune_shape(pf_ctx* c) {
+    switch (c->precision) {
+        case PF_FP64: return tune_shape_t<double, MODE_FP64>(c);
+        case PF_FP32: return tune_shape_t<float, MODE_FP32>(c);
+        default: return tune_shape_t<float, MODE_FP32C>(c);
+    }
 }
 
 template <class R, int MODE, bool SPONGE, int TJ, int KZ>

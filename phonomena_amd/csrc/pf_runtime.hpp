@@ -22,6 +22,7 @@ struct pf_ctx {
     long long plane = 0, cells = 0;  // ny*pz, nx*ny*pz
     int precision = 0, kernel = 0, tile_j = 4, tile_x = 0, tile_z = 64;
     int pair_j = 0, pair_z = 0;  // tile of the two-step sweep (PF_KERNEL_TILED2)
+    bool shape_open = false;     // tile shape left to measurement at creation
     size_t esz = 8;
     bool sponge = false;
     double dt = 0;

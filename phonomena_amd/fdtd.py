@@ -187,10 +187,21 @@ class MeshSpec:
         return nx * ny * nz
 
 
-class Context:
-    """One mesh resident on the GPU (a pf_ctx)."""
+# tile shape a context takes without measuring (rows along y, lanes along z)
+STATIC_TILES = {"fp64": (8, 32), "fp32": (16, 64), "fp32c": (16, 64)}
 
-    def __init__(self, spec, precision="fp64", kernel="auto", tile_j=0, tile_x=0, tile_z=0):
+
+class Context:
+    """One mesh resident on the GPU (a pf_ctx).
+
+    With tile_j = tile_z = 0 the one-step sweep's tile shape is measured at
+    creation (pf_create times each built shape on the new, all-zero fields);
+    tune=False takes a fixed shape instead, for contexts that only step
+    inside a batch (which has its own tiles)."""
+
+    def __init__(self, spec, precision="fp64", kernel="auto", tile_j=0, tile_x=0, tile_z=0, tune=True):
+        if not tune and not tile_j and not tile_z:
+            tile_j, tile_z = STATIC_TILES[precision]
         self.lib = load()
         self.spec = spec
         self.precision = precision

@@ -100,7 +100,9 @@ class Server:
         self.device_ids = device_ids
         self.tile = tile
         self.spec_fn = spec_fn or self._spec
-        self.context_factory = context_factory or (lambda spec: fdtd.Context(spec, precision=self.precision))
+        # served contexts only step inside the batch: no shape measurement
+        self.context_factory = context_factory or (lambda spec: fdtd.Context(spec, precision=self.precision,
+                                                                             tune=False))
         self.batch_factory = batch_factory or (lambda: fdtd.Batch(self.precision))
         self.clock = clock
 

@@ -97,3 +97,34 @@ def test_tiled_sponge_equals_naive(precision):
     b = _fields(spec, precision, "tiled")
     for f in a:
         assert np.array_equal(a[f], b[f]), f
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp32c"])
+def test_measured_shape_starts_clear_and_matches_a_fixed_shape(precision):
+    """A context whose tile shape is left open times the built shapes at
+    creation on its own fields: it must start at step 0 with every level
+    and the probe ring clear, and step to the same bits as a context with a
+    fixed shape."""
+    x, y, z, ids, rows, dt, wave = uniform_crystal((64, 48, 40), 1.0, steps=30, seed=5)
+    spec = spec_from(x, y, z, ids, rows, dt, wave, probe=(0, 4, 8, 0, 4, 8), every=5)
+    ctx = fdtd.Context(spec, precision=precision)
+    try:
+        kernel, tj, tz = ctx.kernel_info()
+        assert kernel == "tiled"
+        built = {(8, 32), (16, 32), (8, 64)} if precision == "fp64" else {(16, 64), (8, 64), (32, 32), (16, 32),
+                                                                           (8, 32)}
+        assert (tj, tz) in built
+        assert ctx.steps_done == 0
+        for f in FIELDS:
+            assert not np.any(ctx.get(f)), f
+        assert ctx.probes().shape[0] == 0
+        ctx.step(30)
+        ctx.sync()
+        got = {f: ctx.get(f) for f in FIELDS}
+        got["probes"] = ctx.probes()
+    finally:
+        ctx.close()
+    tj0, tz0 = fdtd.STATIC_TILES[precision]
+    want = _fields(spec, precision, "tiled", tile_j=tj0, tile_z=tz0, steps=30)
+    for f in want:
+        assert np.array_equal(got[f], want[f]), f
