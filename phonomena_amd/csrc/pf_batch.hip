@@ -534,13 +534,13 @@ extern "C" int pf_batch_autotune(pf_batch* b, int32_t trial_steps, int32_t* tile
     if (b->members.empty()) return fail(PF_ERR_STATE, "batch has no members");
     struct Cand { int tj, tz, tx; };
     std::vector<Cand> cands;
-    const int xs[] = {16, 32, 64};
+    const int xs[] = {16, 32, 64, 128};
     if (b->precision == PF_FP64) {
         for (int tx : xs) { cands.push_back({8, 32, tx}); cands.push_back({16, 32, tx}); cands.push_back({8, 64, tx}); }
     } else {
         for (int tx : xs) {
             cands.push_back({16, 64, tx}); cands.push_back({8, 64, tx});
-            cands.push_back({32, 32, tx}); cands.push_back({16, 32, tx});
+            cands.push_back({32, 32, tx}); cands.push_back({16, 32, tx}); cands.push_back({8, 32, tx});
         }
     }
     // every candidate runs 1 untimed + trial_steps timed steps of real work

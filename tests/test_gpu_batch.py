@@ -96,9 +96,9 @@ def test_batch_autotune_keeps_results():
     ctxs = [fdtd.Context(s, precision="fp32") for s in specs]
     for c in ctxs:
         b.add(c)
-    best = b.autotune(trial_steps=1)   # 12 shapes x 2 real steps each
+    best = b.autotune(trial_steps=1)   # 20 shapes x 2 real steps each
     assert best["tile_z"] in (32, 64) and best["ms_per_step"] > 0
-    b.step(60 - 24)
+    b.step(60 - 40)
     b.sync()
     for i, c in enumerate(ctxs):
         _same(ref[i], _state(c), ("tuned", i))
