@@ -97,11 +97,15 @@ typedef struct pf_desc {
                                   64, 8/16/32 with tile_z 32; two-step: 16
                                   with tile_z 64, 32 with tile_z 32) or per
                                   wavefront (strip: 4/8); 0 = default          */
-    int32_t tile_x;            /* x planes per block (0 = automatic)           */
+    int32_t tile_x;            /* x planes per block (0 = automatic: sized so
+                                  the blocks fill whole rounds of the CUs)    */
     int32_t tile_z;            /* tiled: lanes along z per tile row, 64 (one
                                   row per wavefront) or 32 (two rows per
-                                  wavefront); 0 = the one with fewer idle
-                                  lanes for this mesh                          */
+                                  wavefront).  tile_j = tile_z = 0 with the
+                                  one-step tiled sweep: pf_create times each
+                                  built shape on the new (zero) fields and
+                                  keeps the fastest (identical bits); else 0
+                                  = the one with fewer idle lanes             */
     /* Material map from an inclusion list (used when ids is NULL), the
        device form of Grid.inclusionIndices + Material.setConstants
        (grid.py:158-175, material.py:55-63): node (i, j, k) takes index 1
