@@ -196,10 +196,10 @@ int pf_profile(pf_ctx *ctx, int64_t nsteps, double *sweep_ms, double *face_ms);
 
 /*
  * Batches (no counterpart in the reference, whose solver steps one mesh):
- * many contexts of one precision stepped together.  A batched step is four
+ * many contexts of one precision stepped together.  A batched step is three
  * launches whatever the member count: one persistent sweep over a combined
- * tile work-list of all members, then all members' boundary faces, probes
- * and step counters; the step loop is captured in a HIP graph.  Each member
+ * tile work-list of all members, then all members' boundary faces, then
+ * their probes and step counters; the step loop is captured in a HIP graph.  Each member
  * stops at its own step budget (wave_len) while the others go on, and its
  * fields are bit-identical to stepping it alone.  While a context is a
  * member, pf_step/pf_set_field/pf_profile on it fail (PF_ERR_STATE) and

@@ -480,7 +480,7 @@ template <class R, int MODE>
 static int launch_faces_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
     const long long n = face_total(c->nx, c->ny, c->nz);
     const unsigned g = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL((k_faces<R, MODE>), dim3(g), dim3(256), 0, s, m);
+    hipLaunchKernelGGL((k_faces<R, MODE>), dim3(g), dim3(256), 0, s, m, n);
     HIPCHK(hipGetLastError());
     return PF_OK;
 }

@@ -1,12 +1,12 @@
 // Batched stepping: many contexts of one precision stepped together.
 //
-// Per step the batch issues four launches whatever its size:
+// Per step the batch issues three launches whatever its size:
 //   1. one persistent sweep over a combined tile work-list of all members
 //      (the same per-tile code as the single-mesh tiled sweep, so a member's
 //      fields are bit-identical to stepping it alone);
 //   2. one launch over all members' boundary-face cells;
-//   3. one launch recording every member's probe frame;
-//   4. one launch advancing every member's device step counter.
+//   3. one launch recording every member's probe frame, whose last block to
+//      finish advances every member's device step counter.
 // The step index of a member lives on the device, and which of its two
 // levels is current follows from it, so K steps are captured once in a HIP
 // graph and replayed.  A member whose step counter reaches its source-table
@@ -117,45 +117,58 @@ __global__ __launch_bounds__(TJ * KZ, (MODE == MODE_FP32 ? 4 : 2)) void k_batch_
     }
 }
 
-// Face cells of all members: block g belongs to the member m with
-// fblk[m] <= g < fblk[m + 1].
+// Face block g of a batched step: its member and its index among that
+// member's face blocks (one table load instead of a search).
+struct FaceBlock {
+    int member, block;
+};
+
+// Face cells of all members, BATCH_FACE_CELLS per block (four per thread).
+constexpr int BATCH_FACE_CELLS = 1024;
 template <class R, int MODE>
-__global__ __launch_bounds__(256) void k_batch_faces(const BMember<R>* __restrict__ mem, const long long* __restrict__ fblk,
-                                                     int nmem) {
-    const long long g = blockIdx.x;
-    int lo = 0, hi = nmem - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) / 2;
-        if (fblk[mid] <= g) lo = mid; else hi = mid - 1;
-    }
-    const int mi = __builtin_amdgcn_readfirstlane(lo);
+__global__ __launch_bounds__(256) void k_batch_faces(const BMember<R>* __restrict__ mem,
+                                                     const FaceBlock* __restrict__ fmap) {
+    const FaceBlock fb = load_const(&fmap[blockIdx.x]);
     CMember<R>* cm = (CMember<R>*)mem;
-    const long long t = (g - fblk[mi]) * 256 + threadIdx.x;
-    if (t < cm[mi].faces) {
-        const Mesh<R> m = current_view(mem, mi);
-        faces_one<R, MODE>(m, t);
-    }
+    const Mesh<R> m = current_view(mem, fb.member);
+    faces_run<R, MODE, BATCH_FACE_CELLS / 256>(m, (long long)fb.block * BATCH_FACE_CELLS, cm[fb.member].faces);
 }
 
+// Block i records member i's probe frame.  Every block then counts itself
+// done; the last one (which sees every other block's count, and so every
+// block's reads of its step counter, before it) advances all members' step
+// counters, the batch's step parity, and resets the count for the next
+// step.  No block waits for another.
 template <class R>
-__global__ __launch_bounds__(256) void k_batch_probe(const BMember<R>* __restrict__ mem) {
+__global__ __launch_bounds__(256) void k_batch_probe(const BMember<R>* __restrict__ mem,
+                                                     long long* const* __restrict__ steps,
+                                                     const long long* __restrict__ budget, int nmem,
+                                                     long long* __restrict__ bstep, int* __restrict__ done) {
+    __shared__ int s_last;
     CMember<R>* cm = (CMember<R>*)mem;
     const int i = (int)blockIdx.x;
-    if (cm[i].probe[2] <= 0) return;
-    const Mesh<R> m = current_view(mem, i);
-    probe_record<R>(m.n2, m.nl2, m.plane, m.pz, cm[i].probe[0], cm[i].probe[1], cm[i].probe[2], cm[i].probe[3],
-                    cm[i].probe[4], cm[i].probe[5], cm[i].ring, cm[i].cap, cm[i].frame, *cm[i].step, cm[i].every,
-                    cm[i].budget);
-}
-
-static __global__ __launch_bounds__(256) void k_batch_advance(long long* const* __restrict__ steps,
-                                                              const long long* __restrict__ budget, int nmem,
-                                                              long long* __restrict__ bstep) {
-    for (int i = threadIdx.x; i < nmem; i += blockDim.x) {
-        const long long v = *steps[i] + 1;
-        *steps[i] = v < budget[i] ? v : budget[i];
+    if (cm[i].probe[2] > 0) {
+        const Mesh<R> m = current_view(mem, i);
+        probe_record<R>(m.n2, m.nl2, m.plane, m.pz, cm[i].probe[0], cm[i].probe[1], cm[i].probe[2],
+                        cm[i].probe[3], cm[i].probe[4], cm[i].probe[5], cm[i].ring, cm[i].cap, cm[i].frame,
+                        *cm[i].step, cm[i].every, cm[i].budget);
     }
-    if (threadIdx.x == 0) *bstep += 1;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        s_last = atomicAdd(done, 1) == nmem - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    for (int k = threadIdx.x; k < nmem; k += blockDim.x) {
+        const long long v = *steps[k] + 1;
+        *steps[k] = v < budget[k] ? v : budget[k];
+    }
+    if (threadIdx.x == 0) {
+        *bstep += 1;
+        *done = 0;
+    }
 }
 
 }  // namespace pf
@@ -176,7 +189,8 @@ struct pf_batch {
     int* d_qbeg = nullptr;
     int* d_ctr = nullptr;
     long long* d_bstep = nullptr;
-    long long* d_fblk = nullptr;
+    FaceBlock* d_fmap = nullptr;
+    size_t fmap_cap = 0;
     long long** d_steps = nullptr;
     long long* d_budget = nullptr;
     size_t nmem_cap = 0;
@@ -202,11 +216,11 @@ int free_graph(pf_batch* b) {
 
 void free_tables(pf_batch* b) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
-    F(b->d_mem); F(b->d_items); F(b->d_qbeg); F(b->d_ctr); F(b->d_bstep); F(b->d_fblk);
+    F(b->d_mem); F(b->d_items); F(b->d_qbeg); F(b->d_ctr); F(b->d_bstep); F(b->d_fmap);
     F(b->d_steps); F(b->d_budget);
     b->d_mem = nullptr; b->d_items = nullptr; b->d_qbeg = nullptr; b->d_ctr = nullptr; b->d_bstep = nullptr;
-    b->d_fblk = nullptr; b->d_steps = nullptr; b->d_budget = nullptr;
-    b->mem_bytes = b->items_cap = b->nmem_cap = 0;
+    b->d_fmap = nullptr; b->d_steps = nullptr; b->d_budget = nullptr;
+    b->mem_bytes = b->items_cap = b->nmem_cap = b->fmap_cap = 0;
 }
 
 // Default shapes of a batch (the autotuner refines them).
@@ -286,11 +300,17 @@ int build_t(pf_batch* b) {
             const int ib = xc * e.tm.xchunk;
             cost.push_back(std::min(e.tm.xchunk, c->nx - ib) + 2);
         }
-        fblk[(size_t)i + 1] = fblk[(size_t)i] + (e.faces + 255) / 256;
+        fblk[(size_t)i + 1] = fblk[(size_t)i] + (e.faces + BATCH_FACE_CELLS - 1) / BATCH_FACE_CELLS;
         steps[(size_t)i] = c->d_step;
         budget[(size_t)i] = c->wave_len;
     }
     if (items.size() > 0x7fffffff) return fail(PF_ERR_ARG, "batch work-list too long");
+    if (fblk[(size_t)nmem] > 0x7fffffffLL) return fail(PF_ERR_ARG, "batch has too many face cells");
+    std::vector<FaceBlock> fmap;
+    fmap.reserve((size_t)fblk[(size_t)nmem]);
+    for (int i = 0; i < nmem; i++)
+        for (long long g = fblk[(size_t)i]; g < fblk[(size_t)i + 1]; g++)
+            fmap.push_back({i, (int)(g - fblk[(size_t)i])});
     // 8 contiguous runs of (nearly) equal plane count
     long long total = 0;
     for (long long v : cost) total += v;
@@ -307,28 +327,32 @@ int build_t(pf_batch* b) {
     }
     // (re)allocate and upload
     const size_t mb = sizeof(BMember<R>) * (size_t)nmem;
-    if (b->mem_bytes < mb || b->items_cap < items.size() || b->nmem_cap < (size_t)nmem || !b->d_qbeg) {
+    if (b->mem_bytes < mb || b->items_cap < items.size() || b->nmem_cap < (size_t)nmem || b->fmap_cap < fmap.size() ||
+        !b->d_qbeg) {
         free_tables(b);
         HIPCHK(hipMalloc(&b->d_mem, mb));
         HIPCHK(hipMalloc((void**)&b->d_items, sizeof(WorkItem) * std::max<size_t>(items.size(), 1)));
         HIPCHK(hipMalloc((void**)&b->d_qbeg, sizeof(int) * (NXCD + 1)));
-        HIPCHK(hipMalloc((void**)&b->d_ctr, sizeof(int) * 2 * NXCD));
+        // two steps' queue counters, then the probe launch's done count
+        HIPCHK(hipMalloc((void**)&b->d_ctr, sizeof(int) * (2 * NXCD + 1)));
         HIPCHK(hipMalloc((void**)&b->d_bstep, sizeof(long long)));
-        HIPCHK(hipMalloc((void**)&b->d_fblk, sizeof(long long) * ((size_t)nmem + 1)));
+        HIPCHK(hipMalloc((void**)&b->d_fmap, sizeof(FaceBlock) * std::max<size_t>(fmap.size(), 1)));
         HIPCHK(hipMalloc((void**)&b->d_steps, sizeof(long long*) * (size_t)nmem));
         HIPCHK(hipMalloc((void**)&b->d_budget, sizeof(long long) * (size_t)nmem));
         b->mem_bytes = mb;
         b->items_cap = std::max<size_t>(items.size(), 1);
         b->nmem_cap = (size_t)nmem;
+        b->fmap_cap = std::max<size_t>(fmap.size(), 1);
     }
     HIPCHK(hipStreamSynchronize(b->stream));
     HIPCHK(hipMemcpy(b->d_mem, mem.data(), mb, hipMemcpyHostToDevice));
     if (!items.empty())
         HIPCHK(hipMemcpy(b->d_items, items.data(), sizeof(WorkItem) * items.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b->d_qbeg, qbeg.data(), sizeof(int) * (NXCD + 1), hipMemcpyHostToDevice));
-    HIPCHK(hipMemset(b->d_ctr, 0, sizeof(int) * 2 * NXCD));
+    HIPCHK(hipMemset(b->d_ctr, 0, sizeof(int) * (2 * NXCD + 1)));
     HIPCHK(hipMemset(b->d_bstep, 0, sizeof(long long)));
-    HIPCHK(hipMemcpy(b->d_fblk, fblk.data(), sizeof(long long) * ((size_t)nmem + 1), hipMemcpyHostToDevice));
+    if (!fmap.empty())
+        HIPCHK(hipMemcpy(b->d_fmap, fmap.data(), sizeof(FaceBlock) * fmap.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b->d_steps, steps.data(), sizeof(long long*) * (size_t)nmem, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b->d_budget, budget.data(), sizeof(long long) * (size_t)nmem, hipMemcpyHostToDevice));
     for (int i = 0; i < nmem; i++) {
@@ -367,7 +391,7 @@ void launch_sweep_s(pf_batch* b, int tj, int tz) {
     else launch_sweep<R, MODE, SPONGE, 8, 32>(b);
 }
 
-// Issue one batched step (four launches) on the batch stream.
+// Issue one batched step (three launches) on the batch stream.
 template <class R, int MODE>
 int issue_step_t(pf_batch* b) {
     int tj, tz, tx;
@@ -377,10 +401,10 @@ int issue_step_t(pf_batch* b) {
     else launch_sweep_s<R, MODE, false>(b, tj, tz);
     if (b->face_blocks > 0)
         hipLaunchKernelGGL((k_batch_faces<R, MODE>), dim3((unsigned)b->face_blocks), dim3(256), 0, b->stream,
-                           (const BMember<R>*)b->d_mem, (const long long*)b->d_fblk, nmem);
-    hipLaunchKernelGGL((k_batch_probe<R>), dim3((unsigned)nmem), dim3(256), 0, b->stream, (const BMember<R>*)b->d_mem);
-    hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(256), 0, b->stream, (long long* const*)b->d_steps,
-                       (const long long*)b->d_budget, nmem, b->d_bstep);
+                           (const BMember<R>*)b->d_mem, (const FaceBlock*)b->d_fmap);
+    hipLaunchKernelGGL((k_batch_probe<R>), dim3((unsigned)nmem), dim3(256), 0, b->stream, (const BMember<R>*)b->d_mem,
+                       (long long* const*)b->d_steps, (const long long*)b->d_budget, nmem, b->d_bstep,
+                       b->d_ctr + 2 * NXCD);
     HIPCHK(hipGetLastError());
     return PF_OK;
 }

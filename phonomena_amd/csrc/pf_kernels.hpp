@@ -365,11 +365,10 @@ PF_DEV void face_write(const Mesh<R>& m, long long t, R w) {
     put<Q, MODE>(m, i, j, k, final_value<Q, MODE>(m, i, j, k, w));
 }
 
+// Face cell t (all three components' cells numbered one after another) of a
+// step whose source value is w.
 template <class R, int MODE>
-PF_DEV void faces_one(const Mesh<R>& m, long long t) {
-    const long long tt = step_index(m);
-    if (tt >= m.nsteps) return;
-    const R w = (R)m.wave[tt];
+PF_DEV void faces_one(const Mesh<R>& m, long long t, R w) {
     const long long e0 = face_count<0>(m.nx, m.ny, m.nz);
     const long long e1 = face_count<1>(m.nx, m.ny, m.nz);
     const long long e2 = face_count<2>(m.nx, m.ny, m.nz);
@@ -380,9 +379,25 @@ PF_DEV void faces_one(const Mesh<R>& m, long long t) {
     if (t < e2) face_write<2, MODE>(m, t, w);
 }
 
+// Face cells [t0, t0 + 256 * PER) of view m, PER per thread of a
+// 256-thread block; the step index and source value are read once.  One
+// mesh's faces run best with one cell per thread (more blocks in flight);
+// the batch, whose blocks first read their member's tables, with four.
+template <class R, int MODE, int PER>
+PF_DEV void faces_run(const Mesh<R>& m, long long t0, long long total) {
+    const long long tt = step_index(m);
+    if (tt >= m.nsteps) return;
+    const R w = (R)m.wave[tt];
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+        const long long t = t0 + r * 256 + threadIdx.x;
+        if (t < total) faces_one<R, MODE>(m, t, w);
+    }
+}
+
 template <class R, int MODE>
-__global__ __launch_bounds__(256) void k_faces(Mesh<R> m) {
-    faces_one<R, MODE>(m, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+__global__ __launch_bounds__(256) void k_faces(Mesh<R> m, long long total) {
+    faces_run<R, MODE, 1>(m, (long long)blockIdx.x * 256, total);
 }
 
 inline long long face_total(int nx, int ny, int nz) {
