@@ -383,8 +383,9 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
         const size_t pb = (size_t)c->cap * d->probe[2] * d->probe[5] * sizeof(double);
         HIPCHK(hipMalloc((void**)&c->ring, pb));
         HIPCHK(hipMemset(c->ring, 0, pb));
-        HIPCHK(hipMalloc((void**)&c->d_frame, sizeof(long long)));
-        HIPCHK(hipMemset(c->d_frame, 0, sizeof(long long)));
+        // frame count, then the probe launch's done count (an int)
+        HIPCHK(hipMalloc((void**)&c->d_frame, 2 * sizeof(long long)));
+        HIPCHK(hipMemset(c->d_frame, 0, 2 * sizeof(long long)));
     }
     HIPCHK(hipMalloc((void**)&c->e_out, 2 * sizeof(double)));
     c->e_blocks = 1024;
@@ -492,10 +493,11 @@ static int launch_probe_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
     // the steps that record a frame (the kernel checks it again)
     if (!m.step_base && (m.step_off >= m.nsteps || (m.step_off + 1) % c->every != 0)) return PF_OK;
     // newest level after this step = the n* buffers of the view
-    hipLaunchKernelGGL((k_probe<R>), dim3(1), dim3(256), 0, s, (const R*)m.n2, (const lo_t*)m.nl2, 0,
-                       c->plane, c->pz, c->probe[0], c->probe[1], c->probe[2], c->probe[3], c->probe[4],
-                       c->probe[5], c->ring, c->cap, c->d_frame, m.step_base, m.step_off, c->every,
-                       m.nsteps);
+    const int pts = c->probe[2] * c->probe[5];
+    hipLaunchKernelGGL((k_probe<R>), dim3((unsigned)((pts + 255) / 256)), dim3(256), 0, s, (const R*)m.n2,
+                       (const lo_t*)m.nl2, c->plane, c->pz, c->probe[0], c->probe[1], c->probe[2], c->probe[3],
+                       c->probe[4], c->probe[5], c->ring, c->cap, c->d_frame, (int*)(c->d_frame + 1),
+                       m.step_base, m.step_off, c->every, m.nsteps);
     HIPCHK(hipGetLastError());
     return PF_OK;
 }

@@ -503,17 +503,42 @@ PF_DEV void probe_record(const R* __restrict__ uz, const lo_t* __restrict__ uzl,
     if (threadIdx.x == 0) *frame = f + 1;
 }
 
+// One probe frame of a single mesh, spread over the blocks of the launch:
+// block b gathers points [256 b, 256 b + 256).  Every block reads the frame
+// index, writes its points and counts itself done; the last block to count
+// (after every other block has read the index) advances it and clears the
+// count for the next frame.  No block waits for another.
 template <class R>
 __global__ __launch_bounds__(256) void k_probe(const R* __restrict__ uz, const lo_t* __restrict__ uzl,
-                                               int plane_ny_pz_dummy, long long plane, int pz,
-                                               int i0, int si, int ni, int j0, int sj, int nj,
-                                               double* __restrict__ ring, int cap,
-                                               long long* __restrict__ frame,
-                                               const long long* __restrict__ step_base,
-                                               long long step_off, int every, long long nsteps) {
-    (void)plane_ny_pz_dummy;
+                                               long long plane, int pz, int i0, int si, int ni, int j0, int sj,
+                                               int nj, double* __restrict__ ring, int cap,
+                                               long long* __restrict__ frame, int* __restrict__ done,
+                                               const long long* __restrict__ step_base, long long step_off,
+                                               int every, long long nsteps) {
+    __shared__ int s_last;
     const long long tt = (step_base ? *step_base : 0) + step_off;  // step just completed
-    probe_record<R>(uz, uzl, plane, pz, i0, si, ni, j0, sj, nj, ring, cap, frame, tt, every, nsteps);
+    if (tt >= nsteps || (tt + 1) % every != 0) return;             // the same for every block
+    const long long f = *frame;
+    double* out = ring + (f % cap) * (long long)ni * nj;
+    const int p = (int)blockIdx.x * 256 + (int)threadIdx.x;
+    if (p < ni * nj) {
+        const int a = p / nj, b = p % nj;
+        const long long id = (long long)(i0 + si * a) * plane + (long long)(j0 + sj * b) * pz;
+        const R h = uz[id];
+        double v = (double)h;
+        if (uzl) v += (double)lo_decode(h, uzl[id]);
+        out[p] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        s_last = atomicAdd(done, 1) == (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (s_last && threadIdx.x == 0) {
+        *frame = f + 1;
+        *done = 0;
+    }
 }
 
 // Tiny kernel: advance a device step counter after a captured chunk.

@@ -51,7 +51,7 @@ struct pf_ctx {
     int probe[6] = {};
     int every = 0, cap = 0;
     double* ring = nullptr;
-    long long* d_frame = nullptr;
+    long long* d_frame = nullptr;  // [0] frames recorded; [1], as an int: blocks done in the probe launch
     double* e_out = nullptr;
     double* e_partials = nullptr;
     int e_blocks = 0;
