@@ -128,3 +128,16 @@ def test_measured_shape_starts_clear_and_matches_a_fixed_shape(precision):
     want = _fields(spec, precision, "tiled", tile_j=tj0, tile_z=tz0, steps=30)
     for f in want:
         assert np.array_equal(got[f], want[f]), f
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp32c"])
+def test_unmeasured_context_takes_the_fixed_shape(precision):
+    """tune=False (contexts that only step inside a batch) skips the shape
+    measurement and runs the fixed default shape."""
+    x, y, z, ids, rows, dt, wave = uniform_crystal((40, 32, 24), 1.0, steps=4, seed=1)
+    spec = spec_from(x, y, z, ids, rows, dt, wave)
+    ctx = fdtd.Context(spec, precision=precision, tune=False)
+    try:
+        assert ctx.kernel_info() == ("tiled",) + fdtd.STATIC_TILES[precision]
+    finally:
+        ctx.close()
