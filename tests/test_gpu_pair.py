@@ -88,6 +88,18 @@ def test_pair_sponge_equals_tiled(precision):
     _same(a, b)
 
 
+@pytest.mark.parametrize("cells", [(7, 7, 7), (7, 9, 8), (12, 7, 10)])
+@pytest.mark.parametrize("precision", ["fp32", "fp32c"])
+def test_pair_smallest_meshes(cells, precision):
+    # 8 nodes on an axis: the band's planes nx-3, nx-2 and rows 1, ny-3,
+    # ny-2 leave the pass two rows and three planes of its own
+    x, y, z, ids, rows, dt, wave = uniform_crystal(cells, 1.0, steps=24, seed=4)
+    spec = spec_from(x, y, z, ids, rows, dt, wave, probe=(0, 1, cells[0] + 1, 0, 1, cells[1] + 1), every=2)
+    a = _run(spec, precision, "tiled", [24])
+    for tj, tz in ((16, 64), (32, 32)):
+        _same(a, _run(spec, precision, "tiled2", [5, 19], tile_j=tj, tile_z=tz))
+
+
 def test_pair_rejects_fp64_and_other_tiles():
     x, y, z, ids, rows, dt, wave = uniform_crystal(
         (20, 20, 20), 1.0, incl=(19300.0, 2.0e11, 1.6e11, 6, 2), steps=4, seed=1)
