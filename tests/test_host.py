@@ -281,3 +281,36 @@ def test_cancel_from_another_thread(monkeypatch):
     t.join(10)
     assert not t.is_alive()
     assert s.ctx.n < 10 ** 6
+
+
+class _DescLib:
+    """Stand-in for the library that records the descriptor pf_create gets."""
+
+    def __init__(self):
+        self.desc = None
+
+    def pf_create(self, dptr, hptr):
+        d = dptr._obj
+        self.desc = (d.kernel, d.tile_j, d.tile_z, d.tile_x)
+        hptr._obj.value = 1
+        return 0
+
+    def pf_destroy(self, h):
+        return 0
+
+
+@pytest.mark.parametrize("precision,tiles", [("fp32", (16, 64)), ("fp32c", (16, 64)), ("fp64", (8, 32))])
+def test_context_tune_flag_sets_the_descriptor_tiles(monkeypatch, precision, tiles):
+    # tune=False asks for the fixed shape (no measurement at creation); an
+    # explicit shape is passed as given; the default leaves it to pf_create
+    from phonomena_amd.fdtd import MeshSpec
+    lib = _DescLib()
+    monkeypatch.setattr(fdtd, "load", lambda path=None: lib)
+    spec = MeshSpec((6, 6, 6), [np.ones(5)] * 3, [np.ones(4)] * 3, np.array([[1.0] * 13]),
+                    np.zeros((6, 6, 6), np.uint8), 1.0, 1.0, np.zeros(12), np.zeros(4))
+    fdtd.Context(spec, precision=precision).close()
+    assert lib.desc == (fdtd.KERNEL["auto"], 0, 0, 0)
+    fdtd.Context(spec, precision=precision, tune=False).close()
+    assert lib.desc == (fdtd.KERNEL["auto"],) + tiles + (0,)
+    fdtd.Context(spec, precision=precision, tile_j=32, tile_z=32, tune=False).close()
+    assert lib.desc == (fdtd.KERNEL["auto"], 32, 32, 0)
