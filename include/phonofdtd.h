@@ -106,8 +106,9 @@ typedef struct pf_desc {
                                   built shape on the new (zero) fields and
                                   keeps the fastest (identical bits); else 0
                                   = the one with fewer idle lanes.  A two-
-                                  step context always measures the shape of
-                                  its one-step sweep (odd and single steps)  */
+                                  step context with tile_j = tile_z = 0
+                                  measures the shape of its one-step sweep
+                                  (odd and single steps) the same way         */
     /* Material map from an inclusion list (used when ids is NULL), the
        device form of Grid.inclusionIndices + Material.setConstants
        (grid.py:158-175, material.py:55-63): node (i, j, k) takes index 1

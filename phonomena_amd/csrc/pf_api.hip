@@ -287,10 +287,10 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
         }
         c->tile_z = kz;
         c->tile_j = req_j > 0 ? req_j : (kz == 64 ? tj64 : tj32);
-        // measured at creation unless the caller gave it (a two-step
-        // context's tile fields give the two-step tile, so its one-step
-        // shape, for odd and single steps, is always measured)
-        c->shape_open = (c->kernel == PF_KERNEL_TILED && d->tile_j == 0 && d->tile_z == 0) || pair;
+        // measured at creation when the caller leaves the tile fields at 0
+        // (for a two-step context: the one-step shape of its odd and single
+        // steps)
+        c->shape_open = (c->kernel == PF_KERNEL_TILED || pair) && d->tile_j == 0 && d->tile_z == 0;
         const bool ok = kz == 64 ? (c->tile_j == 4 || c->tile_j == 8 || c->tile_j == 16)
                                  : (c->tile_j == 8 || c->tile_j == 16 || c->tile_j == 32);
         if (!ok) return fail(PF_ERR_ARG, "tile_j must be 4, 8 or 16 with tile_z 64, 8, 16 or 32 with tile_z 32 (got %d, %d)",
