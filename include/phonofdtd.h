@@ -192,8 +192,10 @@ int pf_dft(const double *in_re, const double *in_im, int64_t batch, int64_t n, i
 /*
  * Timing: run nsteps steps with HIP events around every sweep and face
  * launch on the context's stream; returns the mean duration of each per
- * step (ms).  With PF_KERNEL_TILED2, sweep = the two-step pass and face =
- * the faces, probes and band of both steps, each divided by 2 per pair.
+ * step (ms).  With PF_KERNEL_TILED2, sweep = the two-step pass and the
+ * band (the level n+2 cells next to the x and y faces, which the pass
+ * leaves) and face = the faces and probes of both steps, each divided by 2
+ * per pair.
  */
 int pf_profile(pf_ctx *ctx, int64_t nsteps, double *sweep_ms, double *face_ms);
 

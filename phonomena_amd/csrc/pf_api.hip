@@ -316,7 +316,7 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
     c->nlevels = pair ? 4 : 2;
     for (int l = 0; l < c->nlevels; l++) {
         HIPCHK(hipMalloc(&c->lvlbase[l], lvl_bytes));
-        HIPCHK(hipMemset(c->lvlbase[l], 0, lvl_bytes));
+        HIPCHK(hipMemsetAsync(c->lvlbase[l], 0, lvl_bytes, c->stream));
         char* b = (char*)c->lvlbase[l];
         for (int q = 0; q < 3; q++) {
             c->lvl[l][q] = b + q * bytes;
@@ -334,9 +334,9 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
     } else if (c->pz == d->nz) {
         HIPCHK(hipMemcpy(c->ids, d->ids, (size_t)c->cells, hipMemcpyHostToDevice));
     } else {
-        HIPCHK(hipMemset(c->ids, 0, (size_t)c->cells));
-        HIPCHK(hipMemcpy2D(c->ids, (size_t)c->pz, d->ids, (size_t)d->nz, (size_t)d->nz,
-                           (size_t)d->nx * d->ny, hipMemcpyHostToDevice));
+        HIPCHK(hipMemsetAsync(c->ids, 0, (size_t)c->cells, c->stream));
+        HIPCHK(hipMemcpy2DAsync(c->ids, (size_t)c->pz, d->ids, (size_t)d->nz, (size_t)d->nz,
+                                (size_t)d->nx * d->ny, hipMemcpyHostToDevice, c->stream));
     }
 
     int rc = c->esz == 8 ? create_tables<double>(c, d) : create_tables<float>(c, d);
@@ -376,7 +376,7 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
         HIPCHK(hipMemcpy(c->wave, d->wave, (size_t)d->wave_len * sizeof(double), hipMemcpyHostToDevice));
     }
     HIPCHK(hipMalloc((void**)&c->d_step, sizeof(long long)));
-    HIPCHK(hipMemset(c->d_step, 0, sizeof(long long)));
+    HIPCHK(hipMemsetAsync(c->d_step, 0, sizeof(long long), c->stream));
 
     // probes
     memcpy(c->probe, d->probe, sizeof c->probe);
@@ -385,16 +385,18 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
         c->cap = d->probe_capacity;
         const size_t pb = (size_t)c->cap * d->probe[2] * d->probe[5] * sizeof(double);
         HIPCHK(hipMalloc((void**)&c->ring, pb));
-        HIPCHK(hipMemset(c->ring, 0, pb));
+        HIPCHK(hipMemsetAsync(c->ring, 0, pb, c->stream));
         // frame count, then the probe launch's done count (an int)
         HIPCHK(hipMalloc((void**)&c->d_frame, 2 * sizeof(long long)));
-        HIPCHK(hipMemset(c->d_frame, 0, 2 * sizeof(long long)));
+        HIPCHK(hipMemsetAsync(c->d_frame, 0, 2 * sizeof(long long), c->stream));
     }
     HIPCHK(hipMalloc((void**)&c->e_out, 2 * sizeof(double)));
     c->e_blocks = 1024;
     HIPCHK(hipMalloc((void**)&c->e_partials, 2 * sizeof(double) * (size_t)c->e_blocks));
+    // the clears above are ordered on the context's own stream; waiting for
+    // it (not the whole device) lets a context be created while other
+    // contexts' or a batch's work runs
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipDeviceSynchronize());  // the blocking memsets above ran on the null stream
     return PF_OK;
 }
 
@@ -507,7 +509,7 @@ static int launch_probe_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
 
 template <class R, int MODE>
 static int step_once_t(pf_ctx* c, hipStream_t s, const long long* base, long long off,
-                       hipEvent_t* ev /* 3 or null */) {
+                       hipEvent_t* ev /* 3 (of the 5 pf_profile gives) or null */) {
     Mesh<R> m = make_view<R>(c, c->cur, c->old, c->old);
     m.step_base = base;
     m.step_off = off;
@@ -630,6 +632,7 @@ static int step_pair_t(pf_ctx* c, hipStream_t s, const long long* base, long lon
     if (ev) HIPCHK(hipEventRecord(ev[1], s));
     if ((rc = launch_faces_t<R, MODE>(c, m1, s))) return rc;
     if ((rc = launch_probe_t<R>(c, m1, s))) return rc;
+    if (ev) HIPCHK(hipEventRecord(ev[2], s));
     const BandMap bm = band_map(c);
     const long long nb = band_blocks(bm);
     if (nb > 0x7fffffffLL) return fail(PF_ERR_ARG, "too many tiles");
@@ -638,9 +641,10 @@ static int step_pair_t(pf_ctx* c, hipStream_t s, const long long* base, long lon
     else
         hipLaunchKernelGGL((k_band<R, MODE, false>), dim3((unsigned)nb), dim3(64, 4), 0, s, m2, bm.j, bm.i);
     HIPCHK(hipGetLastError());
+    if (ev) HIPCHK(hipEventRecord(ev[3], s));
     if ((rc = launch_faces_t<R, MODE>(c, m2, s))) return rc;
     if ((rc = launch_probe_t<R>(c, m2, s))) return rc;
-    if (ev) HIPCHK(hipEventRecord(ev[2], s));
+    if (ev) HIPCHK(hipEventRecord(ev[4], s));
     c->old = a1;
     c->cur = a2;
     return PF_OK;
@@ -955,32 +959,39 @@ extern "C" int pf_profile(pf_ctx* c, int64_t nsteps, double* sweep_ms, double* f
     if (c->batch) return fail(PF_ERR_STATE, "context is stepped by its batch");
     if (nsteps < 1) return fail(PF_ERR_ARG, "nsteps < 1");
     if (c->step + nsteps > c->wave_len) return fail(PF_ERR_RANGE, "beyond the source table");
-    // three events per launch group: a single step, or a pair of steps of
-    // the two-step sweep (sweep time = the two-step pass, the rest = faces,
-    // probes and band of both steps)
-    std::vector<hipEvent_t> ev((size_t)nsteps * 3);
+    // events per launch group, a single step (0 | sweep | 1 | faces, probe
+    // | 2) or a pair of steps of the two-step sweep (0 | pass | 1 | level
+    // n+1 faces, probe | 2 | band of level n+2 | 3 | level n+2 faces,
+    // probe | 4).  Sweep time = the pass and the band (together they
+    // compute every non-face cell of both levels); the rest = faces and
+    // probes.
+    const int EV = 5;
+    std::vector<hipEvent_t> ev((size_t)nsteps * EV);
+    std::vector<char> pair((size_t)nsteps, 0);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     int rc = PF_OK;
     int64_t groups = 0, s = 0;
     if (c->kernel == PF_KERNEL_TILED2) {
         for (; s + 2 <= nsteps && !rc; s += 2, groups++) {
-            rc = step_pair(c, c->stream, nullptr, c->step, &ev[(size_t)groups * 3]);
+            rc = step_pair(c, c->stream, nullptr, c->step, &ev[(size_t)groups * EV]);
+            pair[(size_t)groups] = 1;
             if (!rc) c->step += 2;
         }
     }
     for (; s < nsteps && !rc; s++, groups++) {
-        rc = step_once(c, c->stream, nullptr, c->step, &ev[(size_t)groups * 3]);
+        rc = step_once(c, c->stream, nullptr, c->step, &ev[(size_t)groups * EV]);
         if (!rc) c->step++;
     }
     if (!rc) {
         HIPCHK(hipStreamSynchronize(c->stream));
         double sw = 0, fc = 0;
-        for (int64_t s = 0; s < groups; s++) {
-            float a = 0, b = 0;
-            HIPCHK(hipEventElapsedTime(&a, ev[(size_t)s * 3], ev[(size_t)s * 3 + 1]));
-            HIPCHK(hipEventElapsedTime(&b, ev[(size_t)s * 3 + 1], ev[(size_t)s * 3 + 2]));
-            sw += a;
-            fc += b;
+        for (int64_t g = 0; g < groups; g++) {
+            const hipEvent_t* e = &ev[(size_t)g * EV];
+            float t[4] = {0, 0, 0, 0};
+            const int n = pair[(size_t)g] ? 4 : 2;
+            for (int k = 0; k < n; k++) HIPCHK(hipEventElapsedTime(&t[k], e[k], e[k + 1]));
+            sw += t[0] + t[2];
+            fc += t[1] + t[3];
         }
         *sweep_ms = sw / nsteps;
         *face_ms = fc / nsteps;

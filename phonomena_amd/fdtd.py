@@ -200,7 +200,9 @@ class Context:
     inside a batch (which has its own tiles)."""
 
     def __init__(self, spec, precision="fp64", kernel="auto", tile_j=0, tile_x=0, tile_z=0, tune=True):
-        if not tune and not tile_j and not tile_z:
+        if not tune and not tile_j and not tile_z and kernel in ("auto", "tiled", "tiled2"):
+            # the fixed shape of the tiled sweeps; the strip and naive
+            # sweeps take their own defaults
             tile_j, tile_z = STATIC_TILES[precision]
         self.lib = load()
         self.spec = spec

@@ -138,26 +138,107 @@ def raster_id(x, y, k, targets):
     return 0
 
 
+MAX_MATERIALS = 256
+# entries of the 6x6 stiffness the stencils read (base_solver.py:329-372):
+# the normal 3x3 block, then the shear diagonal C33 C44 C55
+_ROW_ENTRIES = [(0, 0), (0, 1), (0, 2), (1, 0), (1, 1), (1, 2), (2, 0), (2, 1), (2, 2), (3, 3), (4, 4), (5, 5)]
+
+
+def dense_rows(C, P):
+    """Per-node 13-entry rows (C00 .. C22, C33, C44, C55, rho) of the
+    reference's dense arrays C (nx, ny, nz, 6, 6) and P (nx, ny, nz), as
+    Material.update leaves them (material.py:48-63)."""
+    C = np.asarray(C, dtype=np.float64)
+    P = np.asarray(P, dtype=np.float64)
+    if C.ndim != 5 or C.shape[3:] != (6, 6) or P.shape != C.shape[:3]:
+        raise ValueError("material C must be (nx, ny, nz, 6, 6) and P (nx, ny, nz); got %s and %s"
+                         % (C.shape, P.shape))
+    cols = [C[..., a, b] for a, b in _ROW_ENTRIES] + [P]
+    return np.stack(cols, axis=-1)
+
+
+def material_table(m, nodes):
+    """(rows, ids) of a Material: ``rows`` (n, 13) float64 and ``ids``
+    (nx, ny, nz) uint8 with ids[i, j, k] the row of node (i, j, k).
+
+    This package's Material carries them (``rows()``, ``ids``).  The
+    reference's Material (phonomena/simulation/material.py:22-93) holds the
+    dense per-node arrays ``C`` (nx, ny, nz, 6, 6) and ``P`` instead; then
+    the rows are the distinct 13-entry rows of its nodes (every node holds
+    the primary or the secondary material after setConstants,
+    material.py:55-63, but any per-node assignment of at most 256 distinct
+    materials is taken as it stands), in the order primary, secondary, then
+    the others by first appearance.  The values are copied bit for bit.
+    """
+    ids = getattr(m, "ids", None)
+    if callable(getattr(m, "rows", None)) and ids is not None and tuple(np.shape(ids)) == tuple(nodes):
+        return m.rows(), ids
+    C, P = getattr(m, "C", None), getattr(m, "P", None)
+    if C is None or P is None:
+        raise ValueError("material has neither rows()/ids nor the dense C/P arrays of material.py:48-63")
+    per_node = dense_rows(C, P)
+    if per_node.shape[:3] != tuple(nodes):
+        raise ValueError("material arrays are %s, the grid has %s nodes (call Material.update after "
+                         "Grid.update)" % (per_node.shape[:3], tuple(nodes)))
+    flat = per_node.reshape(-1, 13)
+    # distinct rows by their bytes (exact equality, no tolerance)
+    keys = np.ascontiguousarray(flat).view(np.dtype((np.void, 13 * 8))).ravel()
+    uniq, first, inverse = np.unique(keys, return_index=True, return_inverse=True)
+    if uniq.size > MAX_MATERIALS:
+        raise ValueError("material map has %d distinct materials (at most %d)" % (uniq.size, MAX_MATERIALS))
+    prefer = []
+    for name in ("primary", "secondary"):
+        mat = getattr(m, name, None)
+        if mat is not None and "c" in mat and "p" in mat:
+            c = np.asarray(mat["c"], dtype=np.float64)
+            row = np.array([c[a, b] for a, b in _ROW_ENTRIES] + [float(mat["p"])], dtype=np.float64)
+            hit = np.nonzero(keys[first] == row.view(np.dtype((np.void, 13 * 8)))[0])[0]
+            if hit.size:
+                prefer.append(int(hit[0]))
+    order = []
+    for u in prefer + list(np.argsort(first, kind="stable")):
+        if u not in order:
+            order.append(u)
+    remap = np.empty(uniq.size, dtype=np.uint8)
+    remap[np.array(order, dtype=np.int64)] = np.arange(len(order), dtype=np.uint8)
+    rows = np.ascontiguousarray(flat[first[np.array(order, dtype=np.int64)]])
+    ids = remap[inverse.ravel()].reshape(nodes)
+    # the stencils only ever see rows 0 and 1 of a two-material mesh as
+    # primary and secondary; a mesh of the primary alone still gets a
+    # secondary row so the inclusion list may name it
+    if rows.shape[0] == 1:
+        sec = getattr(m, "secondary", None)
+        if sec is not None and "c" in sec and "p" in sec:
+            c = np.asarray(sec["c"], dtype=np.float64)
+            rows = np.vstack([rows, [c[a, b] for a, b in _ROW_ENTRIES] + [float(sec["p"])]])
+    return rows, np.ascontiguousarray(ids)
+
+
 def mesh_spec(g, m, wave, cfg, device_ids=False):
     """MeshSpec of a solver's grid/material/source under its cfg.  With
     device_ids the material map is rasterized on the device from the grid's
-    inclusion list (m.ids is not needed)."""
+    inclusion list (m.ids is not needed).
+
+    ``m`` is this package's Material or the reference's (dense ``C``/``P``,
+    material.py:22-93): see material_table."""
     nodes = (g.x.size, g.y.size, g.z.size)
     fd, sd = axis_spacings(g)
-    rows = m.rows()
     dt = m.dt
     dt2 = dt ** 2
     raster = None
     if device_ids:
+        rows = m.rows() if callable(getattr(m, "rows", None)) else material_table(m, nodes)[0]
         raster = raster_targets(g)
         id0 = raster_id(raster[0][0], raster[1][0], 0, raster[2])
+        ids = None
     else:
-        id0 = int(m.ids[0, 0, 0])
+        rows, ids = material_table(m, nodes)
+        id0 = int(ids[0, 0, 0])
     mur = mur_coefficients(rows[id0], dt, fd, sd)
     probe = probe_grid(cfg.get("probe"), nodes)
     every = int(cfg.get("record_every", 10))
     steps = len(wave)
     cap = int(cfg.get("probe_capacity", 0)) or max(1, steps // max(every, 1))
-    return MeshSpec(nodes, fd, sd, rows, None if device_ids else m.ids, dt, dt2, mur, wave,
+    return MeshSpec(nodes, fd, sd, rows, ids, dt, dt2, mur, wave,
                     sponge=sponge_tables(nodes, cfg.get("sponge")),
                     probe=probe, record_every=every, probe_capacity=cap, raster=raster)

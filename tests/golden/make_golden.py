@@ -24,7 +24,7 @@ Per-step frames are taken by wrapping the instance's ``time_step`` hook, so
 frame ``t`` is the displacement after the rotation of step ``t`` — the same
 frame the reference's HDF writer stores (``base_solver.py:256-260``).
 
-Usage:  python tests/golden/make_golden.py [REFERENCE_ROOT]
+Usage:  python tests/golden/make_golden.py [REFERENCE_ROOT] [--only nonuniform_deep_40]
 """
 import json
 import os
@@ -34,7 +34,9 @@ import types
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+REF = sys.argv[1] if len(sys.argv) > 1 and not sys.argv[1].startswith("--") else "/root/reference"
+# --only NAME writes that one fixture (the others are left as they are)
+ONLY = sys.argv[sys.argv.index("--only") + 1] if "--only" in sys.argv else None
 
 
 def _install_bridges():
@@ -118,12 +120,74 @@ def _save(name, s, rec, extra):
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
+def _random_start(g, seed):
+    """Seeded non-zero start on every field the reference updates, so the
+    run exercises every cell (faces, edges, free surface) from step 1.  The
+    reference invariants hold: u_new == u at a step's start (time_step
+    copies, base_solver.py:556-571), and the x = 0 wall of uy/uz, which the
+    reference never updates (apply_u_abc has no x = 0 face,
+    base_solver.py:519-554), holds 0.  Returns the start, for the fixture."""
+    rng = np.random.default_rng(seed)
+    start = {}
+    for c in ("x", "y", "z"):
+        a = rng.standard_normal(getattr(g, "u" + c).shape)
+        o = a * 0.75 + 0.25 * rng.standard_normal(a.shape)
+        if c != "x":
+            a[0] = 0.0
+            o[0] = 0.0
+        getattr(g, "u" + c)[...] = a
+        getattr(g, "u" + c + "_new")[...] = a
+        getattr(g, "u" + c + "_old")[...] = o
+        start["start_u" + c] = a.copy()
+        start["start_u" + c + "_old"] = o.copy()
+    return start
+
+
+def case_nonuniform_deep(common, solver_default, steps=40, seed=11):
+    """tests/data/nonuniform.json (graded x/y spacing, 0.3..1.4) with
+    size_z = 10 and max_dz = 0.8 (13 z nodes at 10/12), courant 0.25: at
+    least 8 nodes on every axis and no unit spacing on any, so the tiled,
+    two-step and batched sweeps all run on it.  The inclusions reach
+    z = 5, so they cover a leading part of the depth (7 z nodes for the
+    first, 6 for the others: the reference's rebinding of z in
+    inclusionIndices, grid.py:158-175).  Seeded random start (see
+    _random_start), the json's ricker source, 40 steps."""
+    import tempfile
+    cfg = json.load(open(os.path.join(REF, "tests", "data", "nonuniform.json")))
+    cfg["grid"]["size_z"] = 10
+    cfg["grid"]["max_dz"] = 0.8
+    cfg["simulation"]["courant"] = 0.25
+    with tempfile.NamedTemporaryFile("w", suffix=".json", delete=False) as fh:
+        json.dump(cfg, fh)
+        path = fh.name
+    try:
+        cfg, g, m = common.loadSettings(path)
+    finally:
+        os.unlink(path)
+    s, rec = _frames_solver(solver_default.Solver, 10, 0)
+    s.cfg.update({k: v for k, v in cfg["simulation"]["cfg"].items() if k != "write_mode"})
+    s.cfg["write_mode"] = "off"
+    s.init(g, m, steps)
+    start = _random_start(s.g, seed)
+    s.run()
+    incl = np.array([[t["x"], t["y"], t["z"], t["r"]] for t in cfg["inclusions"]])
+    extra = {"courant": np.float64(cfg["simulation"]["courant"]), "inclusions": incl,
+             "grid_cfg": np.array(json.dumps(cfg["grid"]))}
+    extra.update(start)
+    return s, rec, extra
+
+
 def main():
     _install_bridges()
     sys.path.insert(0, os.path.join(REF, "phonomena"))
     from simulation import base_solver, grid, material  # noqa: E402
     from simulation.solvers import solver_default  # noqa: E402
     import common  # noqa: E402
+
+    if ONLY == "nonuniform_deep_40":
+        s, rec, extra = case_nonuniform_deep(common, solver_default)
+        _save("nonuniform_deep_40", s, rec, extra)
+        return
 
     # 1. BaseSolver.test(): TestDefaults, 10 steps, default cfg (ricker, f=100).
     s, rec = _frames_solver(solver_default.Solver, 1, 1)

@@ -115,3 +115,73 @@ def uniform_crystal(cells, d, rho=2700.0, lam=5.1e10, mu=2.6e10, incl=None, step
         f = 1.0 / (40 * dt)
     wave = rm.source_table(wave_kind, steps, dt, {"f": f})
     return x, y, z, ids, table, dt, wave
+
+
+class ReferenceStyleMaterial:
+    """Stand-in with only the attributes of the reference's Material
+    (phonomena/simulation/material.py:22-93): ``primary``, ``secondary``
+    (property dicts, ``c`` in Pa after init), their keys, ``c_max``, ``dt``
+    and the dense per-node arrays ``C`` (nx, ny, nz, 6, 6) and ``P``.  No
+    ``rows()`` and no ``ids``: what the reference's loader hands a solver
+    plugin (common.py:141-147)."""
+
+    def __init__(self):
+        self.primary = self.primary_key = None
+        self.secondary = self.secondary_key = None
+        self.c_max = 0.5
+        self.dt = 0
+        self.C = np.zeros((0, 0, 0, 6, 6))
+        self.P = np.zeros((0, 0, 0))
+
+    def init(self, grid, properties):
+        import copy
+        self.grid = grid
+        self.properties = dict(copy.deepcopy(properties))
+        for val in self.properties.values():
+            val["c"] = np.array(val["c"]) * 1e10
+        first = list(self.properties)[0]
+        self.setPrimary(first)
+        self.setSecondary(first)
+
+    def setPrimary(self, key):
+        self.primary, self.primary_key = self.properties[key], key
+
+    def setSecondary(self, key):
+        self.secondary, self.secondary_key = self.properties[key], key
+
+    def setTimeStep(self):
+        g = self.grid
+        h = min(np.amin(g.fdx), np.amin(g.fdy), np.amin(g.fdz)) * g.SI_conversion
+        steps = []
+        for mat in (self.primary, self.secondary):
+            v = max(np.sqrt(mat["c"][0][0] / mat["p"]), np.sqrt(mat["c"][3][3] / mat["p"]))
+            steps.append(self.c_max * h / v)
+        self.dt = min(steps)
+
+    def update(self):
+        g = self.grid
+        shape = (g.x.size, g.y.size, g.z.size)
+        self.C = np.zeros(shape + (6, 6))
+        self.P = np.zeros(shape)
+        self.setTimeStep()
+        self.C[...] = np.array(self.primary["c"])
+        self.P[...] = float(self.primary["p"])
+        for yx, z in g.inclusionIndices():
+            for y, x in yx:
+                self.C[x, y, z] = np.array(self.secondary["c"])
+                self.P[x, y, z] = float(self.secondary["p"])
+
+
+def reference_style_material(grid, d):
+    """ReferenceStyleMaterial of a golden fixture's two materials (stiffness
+    given in units of 1e10 Pa, as the reference's JSON holds it)."""
+    m = ReferenceStyleMaterial()
+    props = {"M0": {"name": "m0", "p": float(d["p_primary"]), "c": (d["C_primary"] / 1e10).tolist()},
+             "M1": {"name": "m1", "p": float(d["p_secondary"]), "c": (d["C_secondary"] / 1e10).tolist()}}
+    m.init(grid, props)
+    # the fixture's stiffness in Pa, bit for bit (no round trip through 1e10)
+    m.properties["M0"]["c"] = np.array(d["C_primary"], dtype=np.float64)
+    m.properties["M1"]["c"] = np.array(d["C_secondary"], dtype=np.float64)
+    m.setPrimary("M0")
+    m.setSecondary("M1")
+    return m

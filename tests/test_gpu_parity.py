@@ -78,7 +78,7 @@ def test_fp64_crystal_two_materials():
     ctx.close()
 
 
-@pytest.mark.parametrize("precision,tol", [("fp32", 2e-4), ("fp32c", 2e-4)])
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-4), ("fp32c", 1e-5)])
 def test_fp32_modes_vs_fp64_oracle(precision, tol):
     x, y, z, ids, rows, dt, wave = uniform_crystal(
         (48, 40, 24), 1.0, incl=(19300.0, 2.0e11, 1.6e11, 16, 5), steps=600, wave_kind="sin", seed=3)

@@ -314,3 +314,101 @@ def test_context_tune_flag_sets_the_descriptor_tiles(monkeypatch, precision, til
     assert lib.desc == (fdtd.KERNEL["auto"],) + tiles + (0,)
     fdtd.Context(spec, precision=precision, tile_j=32, tile_z=32, tune=False).close()
     assert lib.desc == (fdtd.KERNEL["auto"], 32, 32, 0)
+    # the strip and naive sweeps keep their own defaults (no tiled shape
+    # forced on them, which the strip sweep would reject)
+    for k in ("strip", "naive"):
+        fdtd.Context(spec, precision=precision, kernel=k, tune=False).close()
+        assert lib.desc == (fdtd.KERNEL[k], 0, 0, 0)
+
+
+# ------------------------------------------- the reference's own objects --
+
+def _spec_fields(spec):
+    return (spec.nodes, spec.materials, spec.ids, spec.dt, spec.dt2, spec.mur, spec.wave)
+
+
+def _same_spec(a, b):
+    for u, v in zip(_spec_fields(a), _spec_fields(b)):
+        assert np.array_equal(np.asarray(u), np.asarray(v))
+
+
+def test_reference_style_material_drives_init_and_run(monkeypatch):
+    # a Material with only the reference's attributes (dense C/P, no rows()
+    # or ids), as common.loadSettings hands it to a plugin
+    from helpers import ReferenceStyleMaterial
+    monkeypatch.setattr(fdtd, "Context", _FakeContext)
+    ref_m = ReferenceStyleMaterial()
+    ref_m.init(base_solver.TestDefaults.g, base_solver.TestDefaults.properties)
+    assert not hasattr(ref_m, "rows") and not hasattr(ref_m, "ids")
+    s = solver_hip.Solver()
+    s.cfg["write_mode"] = "off"
+    s.init(base_solver.TestDefaults.g, ref_m, 10)
+    s.run()
+    assert s.ctx.n == 10
+    own = solver_hip.Solver()
+    own.cfg["write_mode"] = "off"
+    own.init(base_solver.TestDefaults.g, base_solver.TestDefaults.m, 10)
+    _same_spec(s.ctx.spec, own.ctx.spec)
+
+
+@pytest.mark.parametrize("name", ["default_json_100", "nonuniform_json_60", "nonuniform_deep_40"])
+def test_reference_style_material_gives_fixture_tables(name, monkeypatch):
+    # two materials with inclusions (partial depth in nonuniform_deep_40):
+    # the rows are primary then secondary, the map is the fixture's
+    from helpers import reference_style_material
+    monkeypatch.setattr(fdtd, "Context", _FakeContext)
+    d = golden(name)
+    settings = _settings_from_fixture(d)
+    g = grid.Grid()
+    g.init(**{k: settings["grid"][k] for k in ("size_x", "size_y", "size_z")})
+    for k in ("min_d", "max_dx", "max_dy", "max_dz", "slope"):
+        setattr(g, k, settings["grid"][k])
+    for t in settings["inclusions"]:
+        g.addInclusion(x=t["x"], y=t["y"], z=t["z"], r=t["r"])
+    g.buildMesh()
+    g.update()
+    m = reference_style_material(g, d)
+    m.c_max = float(d["courant"])
+    m.update()
+    rows, ids = problem.material_table(m, (g.x.size, g.y.size, g.z.size))
+    assert np.array_equal(rows, rows_of(d))
+    assert np.array_equal(ids, d["ids"])
+    assert m.dt == d["dt"][()]
+    s = solver_hip.Solver()
+    s.cfg.update(write_mode="off", wave=json.loads(str(d["cfg"]))["wave"])
+    s.init(g, m, 5)
+    assert np.array_equal(s.ctx.spec.materials, rows_of(d)) and np.array_equal(s.ctx.spec.ids, d["ids"])
+
+
+def test_material_table_takes_any_per_node_assignment():
+    # more than two materials, assigned per node: rows are the distinct
+    # per-node rows, and rows[ids] rebuilds every node's entries bit for bit
+    from helpers import ReferenceStyleMaterial
+    rng = np.random.default_rng(3)
+    m = ReferenceStyleMaterial()
+    nodes = (6, 5, 7)
+    base = [rng.uniform(1, 2, (6, 6)) * 1e10 for _ in range(4)]
+    rho = [2700.0, 19300.0, 1.2, 5307.0]
+    pick = rng.integers(0, 4, nodes)
+    pick[0, 0, 0] = 3  # the secondary first in memory: still row 1
+    pick[1, 1, 1] = 0
+    m.C = np.stack(base)[pick]
+    m.P = np.array(rho)[pick]
+    m.primary = {"c": base[0], "p": rho[0]}
+    m.secondary = {"c": base[3], "p": rho[3]}
+    rows, ids = problem.material_table(m, nodes)
+    assert rows.shape == (4, 13) and ids.dtype == np.uint8
+    assert np.array_equal(rows[ids], problem.dense_rows(m.C, m.P))
+    entries = lambda c: [c[a, b] for a, b in problem._ROW_ENTRIES]
+    assert np.array_equal(rows[0][:12], entries(base[0])) and rows[0][12] == rho[0]  # primary
+    assert np.array_equal(rows[1][:12], entries(base[3])) and rows[1][12] == rho[3]  # secondary
+    assert ids[0, 0, 0] == 1 and ids[1, 1, 1] == 0
+
+
+def test_material_table_rejects_mismatched_arrays():
+    from helpers import ReferenceStyleMaterial
+    m = ReferenceStyleMaterial()
+    m.C = np.zeros((4, 4, 4, 6, 6))
+    m.P = np.ones((4, 4, 4))
+    with pytest.raises(ValueError, match="nodes"):
+        problem.material_table(m, (5, 4, 4))
