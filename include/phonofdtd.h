@@ -151,6 +151,19 @@ int pf_kernel_info(pf_ctx *ctx, int32_t *kernel, int32_t *tile_j, int32_t *tile_
 /* Fields in reference shapes, converted to/from double. */
 int pf_get_field(pf_ctx *ctx, int32_t which, double *host, int64_t count);
 /*
+ * pf_get_field without any device allocation.  No counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid it):
+ * pf_get_field stages the packed FP64 field in a buffer it allocates on
+ * first use, this variant in one static array per device of
+ * PF_NOALLOC_FIELD_MAX doubles (32 MiB), held until the process ends.
+ * Fields with more elements are refused (PF_ERR_ARG).  Every call on a
+ * device shares that device's array: calls that overlap on one GPU (from
+ * several host threads) race.  Same result as pf_get_field.
+ */
+#define PF_NOALLOC_FIELD_MAX (4LL * 1024 * 1024)
+int pf_get_field_noalloc(pf_ctx *ctx, int32_t which, double *host, int64_t count);
+/*
  * Asynchronous snapshots of the current level, for the HDF output path
  * (Writer, base_solver.py:72-169, put :256-260): ux, uy, uz in the
  * reference's shapes as FP64, back to back, in pinned host memory.

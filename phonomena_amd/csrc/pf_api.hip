@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -301,6 +302,10 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
     c->dt = d->dt;
     c->nmat = d->n_materials;
     HIPCHK(hipGetDevice(&c->dev));
+    {
+        int rc = noalloc_scratch_init(c->dev);
+        if (rc) return rc;
+    }
     // sizes the x chunks of the tiled sweeps (0 keeps the fixed rule)
     if (hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, c->dev) != hipSuccess) {
         (void)hipGetLastError();
@@ -767,10 +772,11 @@ extern "C" int pf_snapshot(pf_ctx* c, int32_t slot) {
         const unsigned g = (unsigned)((n + 255) / 256);
         if (c->esz == 8)
             hipLaunchKernelGGL((k_pack<double>), dim3(g), dim3(256), 0, c->stream, (const double*)c->lvl[c->cur][q],
-                               (const lo_t*)nullptr, c->plane, c->pz, ex, ey, ez, dev + off, 0, 0.0);
+                               (const lo_t*)nullptr, c->plane, c->pz, ex, ey, ez, dev + off, (const double*)nullptr);
         else
             hipLaunchKernelGGL((k_pack<float>), dim3(g), dim3(256), 0, c->stream, (const float*)c->lvl[c->cur][q],
-                               (const lo_t*)c->lvlo[c->cur][q], c->plane, c->pz, ex, ey, ez, dev + off, 0, 0.0);
+                               (const lo_t*)c->lvlo[c->cur][q], c->plane, c->pz, ex, ey, ez, dev + off,
+                               (const double*)nullptr);
         HIPCHK(hipGetLastError());
         off += n;
     }
@@ -816,35 +822,89 @@ extern "C" int pf_get_ids(pf_ctx* c, uint8_t* host, int64_t count) {
     return PF_OK;
 }
 
-extern "C" int pf_get_field(pf_ctx* c, int32_t which, double* host, int64_t count) {
-    if (!c || !host) return fail(PF_ERR_ARG, "NULL argument");
-    if (which < 0 || which > 5) return fail(PF_ERR_ARG, "bad field %d", which);
-    if (c->batch) HIPCHK(hipStreamSynchronize(c->batch_stream));
+// Field `which` into host[0 .. count): packed to FP64 in the reference's
+// shape into the device staging buffer `stage` (count doubles), then copied.
+static int get_field_via(pf_ctx* c, int32_t which, double* host, int64_t count, double* stage) {
     const int q = which % 3;
     const int lvl = which < 3 ? c->cur : c->old;
     int ex, ey, ez;
     extents(c, q, ex, ey, ez);
     const long long n = (long long)ex * ey * ez;
-    if (count != n) return fail(PF_ERR_ARG, "field %d has %lld elements, got %lld", which, n, (long long)count);
-    int rc = ensure_scratch(c, (size_t)n * sizeof(double));
-    if (rc) return rc;
     // the previous level of uz holds the source value on its wall line, as
     // the reference's u_old does after time_step
-    int subst = (which == PF_UZ_OLD && c->step > 0) ? 1 : 0;
-    double w = 0.0;
-    if (subst) HIPCHK(hipMemcpy(&w, c->wave + (c->step - 1), sizeof(double), hipMemcpyDeviceToHost));
+    const double* wsub = (which == PF_UZ_OLD && c->step > 0) ? c->wave + (c->step - 1) : nullptr;
     unsigned g = (unsigned)((n + 255) / 256);
     if (c->esz == 8)
         hipLaunchKernelGGL((k_pack<double>), dim3(g), dim3(256), 0, c->stream, (const double*)c->lvl[lvl][q],
-                           (const lo_t*)nullptr, c->plane, c->pz, ex, ey, ez, (double*)c->scratch, subst, w);
+                           (const lo_t*)nullptr, c->plane, c->pz, ex, ey, ez, stage, wsub);
     else
         hipLaunchKernelGGL((k_pack<float>), dim3(g), dim3(256), 0, c->stream, (const float*)c->lvl[lvl][q],
-                           (const lo_t*)c->lvlo[lvl][q], c->plane, c->pz, ex, ey, ez, (double*)c->scratch,
-                           subst, w);
+                           (const lo_t*)c->lvlo[lvl][q], c->plane, c->pz, ex, ey, ez, stage, wsub);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(host, c->scratch, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(host, stage, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    (void)count;
     return PF_OK;
+}
+
+static int check_field(pf_ctx* c, int32_t which, const void* host, int64_t count, long long* n_out) {
+    if (!c || !host) return fail(PF_ERR_ARG, "NULL argument");
+    if (which < 0 || which > 5) return fail(PF_ERR_ARG, "bad field %d", which);
+    int ex, ey, ez;
+    extents(c, which % 3, ex, ey, ez);
+    const long long n = (long long)ex * ey * ez;
+    if (count != n) return fail(PF_ERR_ARG, "field %d has %lld elements, got %lld", which, n, (long long)count);
+    *n_out = n;
+    return PF_OK;
+}
+
+extern "C" int pf_get_field(pf_ctx* c, int32_t which, double* host, int64_t count) {
+    long long n = 0;
+    int rc = check_field(c, which, host, count, &n);
+    if (rc) return rc;
+    if (c->batch) HIPCHK(hipStreamSynchronize(c->batch_stream));
+    if ((rc = ensure_scratch(c, (size_t)n * sizeof(double)))) return rc;
+    return get_field_via(c, which, host, count, (double*)c->scratch);
+}
+
+// ------------------------------------------- allocation-free transfer ----
+// One staging array per device for pf_get_field_noalloc, held for the life
+// of the process.  Its name is a host-side stand-in: host code uses only the
+// device address hipGetSymbolAddress gives for the current device, taken
+// once per device when a context is created there (noalloc_scratch_init).
+static __device__ __align__(256) double scratch[PF_NOALLOC_FIELD_MAX];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[MAX_DEVICES] = {};
+}  // namespace
+
+int pf::noalloc_scratch_init(int dev) {
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(PF_ERR_ARG, "device %d beyond %d", dev, MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return PF_OK;
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));  // the current device is dev
+    g_scratch_ptr[dev] = (double*)p;
+    return PF_OK;
+}
+
+extern "C" int pf_get_field_noalloc(pf_ctx* c, int32_t which, double* host, int64_t count) {
+    long long n = 0;
+    int rc = check_field(c, which, host, count, &n);
+    if (rc) return rc;
+    if (n > PF_NOALLOC_FIELD_MAX)
+        return fail(PF_ERR_ARG, "field %d has %lld elements; pf_get_field_noalloc takes at most %lld (use pf_get_field)",
+                    which, n, (long long)PF_NOALLOC_FIELD_MAX);
+    double* stage = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        stage = (c->dev >= 0 && c->dev < MAX_DEVICES) ? g_scratch_ptr[c->dev] : nullptr;
+    }
+    if (!stage) return fail(PF_ERR_STATE, "no staging array on device %d", c->dev);
+    if (c->batch) HIPCHK(hipStreamSynchronize(c->batch_stream));
+    return get_field_via(c, which, host, count, stage);
 }
 
 extern "C" int pf_set_field(pf_ctx* c, int32_t which, const double* host, int64_t count) {

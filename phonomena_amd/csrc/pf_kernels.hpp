@@ -554,7 +554,7 @@ static __global__ void k_advance(long long* counter, long long by, long long cap
 template <class R>
 __global__ void k_pack(const R* __restrict__ hi, const lo_t* __restrict__ lo, long long plane, int pz,
                        int ex, int ey, int ez, double* __restrict__ out,
-                       int subst, double w) {
+                       const double* __restrict__ wsub) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long n = (long long)ex * ey * ez;
     if (t >= n) return;
@@ -566,7 +566,7 @@ __global__ void k_pack(const R* __restrict__ hi, const lo_t* __restrict__ lo, lo
     const R h = hi[id];
     double v = (double)h;
     if (lo) v += (double)lo_decode(h, lo[id]);
-    if (subst && i == 0 && k == 0) v = w;
+    if (wsub && i == 0 && k == 0) v = *wsub;  // the source value on the wall line
     out[t] = v;
 }
 

@@ -95,6 +95,9 @@ int step_once(pf_ctx* c, hipStream_t s, const long long* base, long long off, hi
 // Issue steps tt and tt+1 with the two-step sweep (PF_KERNEL_TILED2); ev as
 // for step_once (the middle event follows the two-step pass).
 int step_pair(pf_ctx* c, hipStream_t s, const long long* base, long long off, hipEvent_t* ev);
+// Device address of the allocation-free staging array on device dev, taken
+// once per device (called at context creation, with dev current).
+int noalloc_scratch_init(int dev);
 
 }  // namespace pf
 

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libphonofdtd.so")
 PF_OK = 0
 ERRORS = {-1: "PF_ERR_ARG", -2: "PF_ERR_HIP", -3: "PF_ERR_STATE", -4: "PF_ERR_NOMEM", -5: "PF_ERR_RANGE"}
 PRECISION = {"fp64": 0, "fp32": 1, "fp32c": 2}
+NOALLOC_FIELD_MAX = 4 * 1024 * 1024  # PF_NOALLOC_FIELD_MAX
 KERNEL = {"auto": 0, "naive": 1, "tiled": 2, "strip": 3, "tiled2": 4}
 FIELDS = {"ux": 0, "uy": 1, "uz": 2, "ux_old": 3, "uy_old": 4, "uz_old": 5}
 
@@ -83,6 +84,7 @@ def load(path=None):
         "pf_kernel_info": ([V, P(I32), P(I32), P(I32)], ctypes.c_int),
         "pf_get_field": ([V, I32, V, I64], ctypes.c_int),
         "pf_get_ids": ([V, V, I64], ctypes.c_int),
+        "pf_get_field_noalloc": ([V, I32, V, I64], ctypes.c_int),
         "pf_snapshot_open": ([V, I32], ctypes.c_int),
         "pf_snapshot": ([V, I32], ctypes.c_int),
         "pf_snapshot_wait": ([V, I32, P(P(D)), P(I64)], ctypes.c_int),
@@ -315,6 +317,16 @@ class Context:
         shape = self._shapes[which.replace("_old", "")]
         out = np.empty(shape, dtype=np.float64)
         check(self.lib.pf_get_field(self.handle, FIELDS[which], out.ctypes.data_as(ctypes.c_void_p), out.size))
+        return out
+
+    def get_noalloc(self, which):
+        """As get(), staged in the library's static per-device array
+        (pf_get_field_noalloc: no device allocation; fields of at most
+        NOALLOC_FIELD_MAX elements)."""
+        shape = self._shapes[which.replace("_old", "")]
+        out = np.empty(shape, dtype=np.float64)
+        check(self.lib.pf_get_field_noalloc(self.handle, FIELDS[which], out.ctypes.data_as(ctypes.c_void_p),
+                                            out.size))
         return out
 
     def set(self, which, arr):
