@@ -178,6 +178,9 @@ def bench_plain(args):
                    "fill_fraction": float(m.ids.mean())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
+                     # the same algorithmic bytes over the whole timed step
+                     # (faces, probes and launch gaps included)
+                     "frac_whole_step": value * bpn / (HBM_PEAK_GBS * 1e9),
                      "traffic": traffic_from_profiles(args.workload, args.precision, kernel),
                      "kernel": SWEEP_KERNEL[kernel], "steps_per_launch": per_launch,
                      "launch_ms": sweep_ms * per_launch, "bytes_per_node": bpn,
@@ -187,6 +190,23 @@ def bench_plain(args):
     }
     ctx.close()
     return rec
+
+
+def bench_other_precisions(args, precisions=("fp32c", "fp64")):
+    """The plain workload in the other arithmetic modes (AUTO's sweep for
+    each): FP32 compensated, and FP64, the reference's own arithmetic
+    (base_solver.py:26), bit-identical to it."""
+    out = {}
+    for prec in precisions:
+        a = argparse.Namespace(**dict(vars(args), precision=prec, kernel="auto", tile_j=0, tile_z=0, tile_x=0))
+        r = bench_plain(a)
+        out[prec] = {"value": r["value"], "unit": r["unit"], "ms_per_step": r["ms_per_step"], "dtype": r["dtype"],
+                     "kernel": r["config"]["kernel"], "tile_j": r["config"]["tile_j"],
+                     "tile_z": r["config"]["tile_z"],
+                     "roofline": {k: r["roofline"][k] for k in ("achieved", "peak", "unit", "frac", "frac_whole_step",
+                                                                 "bytes_per_node", "kernel_bytes_per_node",
+                                                                 "sweep_ms", "face_ms", "traffic")}}
+    return out
 
 
 def bench_setup(workload="plain", precision="fp32"):
@@ -358,13 +378,20 @@ def main():
                     help="steps of the FP32C/FP32 vs FP64 precision case (0 = skip)")
     ap.add_argument("--serve-jobs", type=int, default=48, help="jobs in the served stream")
     ap.add_argument("--no-setup", action="store_true", help="skip the mesh set-up (material map) leg")
+    ap.add_argument("--no-precisions", action="store_true",
+                    help="skip the FP32C / FP64 legs of the plain workload")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain leg (for profiling the other legs alone; the line then lacks value)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design "
               "note (slab decomposition along x), not implemented", file=sys.stderr)
         sys.exit(2)
-    rec = bench_plain(args)
-    if not args.no_stream:
+    rec = {"metric": "cell-updates/s", "value": None, "config": {"nodes": 0}, "roofline": {}} if args.no_plain \
+        else bench_plain(args)
+    if not args.no_plain and not args.no_precisions and args.precision == "fp32":
+        rec["other_precisions"] = bench_other_precisions(args)
+    if not args.no_stream and not args.no_plain:
         rec["roofline"]["stream_peak_gbs"] = stream_peaks(rec["config"]["nodes"])
     if not args.no_batch:
         rec["batched"] = bench_batched(args.batch_steps)

@@ -245,10 +245,13 @@ int pf_batch_destroy(pf_batch *b);
 /*
  * Diagnostics (no counterpart in the reference).  Streams `reads` input and
  * `writes` output float arrays of n elements each through the device `reps`
- * times with a plain coalesced kernel and returns the achieved GB/s and the
+ * times with a coalesced one-pass kernel (each block a contiguous run of
+ * float4 elements, all of a thread's loads issued before their use) in a few
+ * launch shapes (1 or 2 float4 per stream and thread, 256 or 512 threads,
+ * plain or non-temporal accesses) and returns the best achieved GB/s and its
  * mean ms per pass: the practical HBM bandwidth of that mix on this device,
  * and a kernel of known traffic for calibrating hardware counters.  Built
- * mixes (reads/writes): 1/1, 2/1, 6/3, 7/3, 1/0.  Allocates its own buffers.
+ * mixes (reads/writes): 1/1, 2/1, 6/3, 7/3.  Allocates its own buffers.
  */
 int pf_stream_probe(int32_t reads, int32_t writes, int64_t n, int32_t reps, double *gbs, double *ms);
 

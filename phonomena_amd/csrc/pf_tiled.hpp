@@ -273,12 +273,14 @@ PF_DEV double above0(double v) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// The march of one block over its tile and chunk.  FULL: every lane of the
-// tile, halos included, lies at 1 <= k <= nz-4 and 1 <= j <= ny-3, and the
-// chunk at 2 <= ib, ie <= nx-3, so no lane meets the free surface, a range
-// limit or a face, and every range select of the general code is the
-// identity; the FULL variant leaves them out (same bits).
-template <class R, int MODE, bool SPONGE, int TJ, int KZ, bool FULL>
+// The march of one block over its tile and chunk.  FYZ: every lane of the
+// tile, halos included, lies at 1 <= k <= nz-4 and 1 <= j <= ny-3, so no
+// lane meets the free surface, a y or z range limit or face; FX: the chunk
+// lies at 2 <= ib, ie <= nx-3, clear of the x = 0 wall and the x face.
+// Where a flag holds, the range selects it covers are the identity and are
+// left out (same bits); the x conditions left in a FYZ block are uniform
+// over the block (per plane).
+template <class R, int MODE, bool SPONGE, int TJ, int KZ, bool FYZ, bool FX>
 PF_DEV void tiled_march(const Mesh<R>& m, TiledLds<R, TJ, KZ>& L, int kl, int row, int k, int j, int ib, int ie,
                         R w) {
     constexpr bool EXACT = MODE == MODE_FP64;
@@ -292,14 +294,15 @@ PF_DEV void tiled_march(const Mesh<R>& m, TiledLds<R, TJ, KZ>& L, int kl, int ro
     // spacings / operands, selected here once: the per-value formulas are
     // then evaluated without branches and give the same bits as the
     // separate surface expressions (x/1 = x and x - 0 = x exactly).
-    const bool kval = FULL || (k >= 0 && k < m.nz);
-    const bool jval = FULL || (j >= 0 && j < m.ny);
+    constexpr bool FULL = FYZ && FX;
+    const bool kval = FYZ || (k >= 0 && k < m.nz);
+    const bool jval = FYZ || (j >= 0 && j < m.ny);
     const bool valid = kval && jval;
-    const bool ks = FULL || k >= 1;                          // interior in z
-    const bool jn = FULL || (j >= 1 && j <= m.ny - 2);       // staggered y index valid
-    const bool jf = FULL || (j >= 0 && j <= m.ny - 2);       // full y index valid
-    const bool kn = FULL || (k >= 1 && k <= m.nz - 2);
-    const bool kf = FULL || (k >= 0 && k <= m.nz - 2);
+    const bool ks = FYZ || k >= 1;                          // interior in z
+    const bool jn = FYZ || (j >= 1 && j <= m.ny - 2);       // staggered y index valid
+    const bool jf = FYZ || (j >= 0 && j <= m.ny - 2);       // full y index valid
+    const bool kn = FYZ || (k >= 1 && k <= m.nz - 2);
+    const bool kf = FYZ || (k >= 0 && k <= m.nz - 2);
     const bool writer = valid && kl >= 1 && kl <= KZ - 2 && row >= 1 && row <= TJ - 2;
     const R hsy = jn ? m.sy[j - 1] : R(1);
     const R hfy = jf ? m.fy[j] : R(1);
@@ -325,8 +328,8 @@ PF_DEV void tiled_march(const Mesh<R>& m, TiledLds<R, TJ, KZ>& L, int kl, int ro
     const bool wx = writer && jn && k <= m.nz - 2;
     const bool wy = writer && j >= 1 && j <= m.ny - 3 && k <= m.nz - 2;
     const bool wz = writer && jn && k <= m.nz - 3;
-    const bool top_xy = !FULL && k == m.nz - 2;
-    const bool top_z = !FULL && k == m.nz - 3;
+    const bool top_xy = !FYZ && k == m.nz - 2;
+    const bool top_z = !FYZ && k == m.nz - 3;
 
     const Bufs bf = make_bufs<R, COMP>(m);
     // only writer lanes use the previous level
@@ -338,7 +341,7 @@ PF_DEV void tiled_march(const Mesh<R>& m, TiledLds<R, TJ, KZ>& L, int kl, int ro
     // block reads anyway; planes outside the mesh are then zeroed by settle().
     const int pmax = ie < m.nx ? ie : m.nx - 1;
     Col<R> c0 = settle(load_raw<R>(bf, ib >= 1 ? ib - 1 : 0, plane, boff, col), valid && ib >= 1,
-                       !FULL && ib - 1 == 0 && k == 0, w);
+                       !FYZ && ib - 1 == 0 && k == 0, w);
     c0.id *= RB;
     // issue order ra, rb, ob (kept by the scheduling barriers) matches the
     // steady state, so the loop needs no wait for more than it uses
@@ -364,7 +367,7 @@ PF_DEV void tiled_march(const Mesh<R>& m, TiledLds<R, TJ, KZ>& L, int kl, int ro
     auto plane_step = [&](int i, auto Hc, Raw<R>& rin, Old<R>& oin) {
         constexpr int H = decltype(Hc)::value;
         const int ip = i + 1;
-        Col<R> c1 = settle(rin, valid && (FULL || ip < m.nx), !FULL && ip == 0 && k == 0, w);
+        Col<R> c1 = settle(rin, valid && (FX || ip < m.nx), !FYZ && ip == 0 && k == 0, w);
         c1.id *= RB;
         L.su[H][0][row][kl] = c1.x;
         L.su[H][1][row][kl] = c1.y;
@@ -431,11 +434,11 @@ PF_DEV void tiled_march(const Mesh<R>& m, TiledLds<R, TJ, KZ>& L, int kl, int ro
             // value except at the wall source line, whose lo is 0)
             const R cl0 = COMP ? lo_decode(c0.x, oin.cl0) : R(0);
             const R cl1 = COMP ? lo_decode(c0.y, oin.cl1) : R(0);
-            const R cl2 = (!COMP || (!FULL && i == 0 && k == 0)) ? R(0) : lo_decode(c0.z, oin.cl2);
+            const R cl2 = (!COMP || (!FYZ && i == 0 && k == 0)) ? R(0) : lo_decode(c0.z, oin.cl2);
             const R ol0 = COMP ? lo_decode(oin.o0, oin.ol0) : R(0);
             const R ol1 = COMP ? lo_decode(oin.o1, oin.ol1) : R(0);
             const R ol2 = COMP ? lo_decode(oin.o2, oin.ol2) : R(0);
-            const bool wall = !FULL && i == 0;  // uy, uz on the x = 0 wall stay 0
+            const bool wall = !FX && i == 0;  // uy, uz on the x = 0 wall stay 0
             // ux (base_solver.py:441-447, 495-501)
             {
                 const R dv = div3<EXACT>(n1_1 - n1_0, sel(ks, xq0.hf, m.fx0), s6 - s6_jm, hy_n,
@@ -443,7 +446,7 @@ PF_DEV void tiled_march(const Mesh<R>& m, TiledLds<R, TJ, KZ>& L, int kl, int ro
                 const V<R> cur = {c0.x, cl0};
                 const V<R> s0 = leap_from<MODE, SPONGE>(cur, oin.o0, ol0, mget(smat, c1.id, M_COEF), dv, s);
                 const V<R> above = {ux0_kp, COMP ? above0(cl0) : R(0)};
-                if (wx && (FULL || i <= m.nx - 3))
+                if (wx && (FX || i <= m.nx - 3))
                     put_tile<0, MODE>(m, bf, boff, loff, soff, slo, top_xy, s0, cur, above);
             }
             // uy (base_solver.py:449-455, 503-509)
@@ -453,7 +456,7 @@ PF_DEV void tiled_march(const Mesh<R>& m, TiledLds<R, TJ, KZ>& L, int kl, int ro
                 V<R> s0 = leap_from<MODE, SPONGE>(cur, oin.o1, ol1, mget(smat, id_jp, M_COEF), dv, s);
                 if (wall) s0 = {R(0), R(0)};
                 const V<R> above = {uy0_kp, COMP ? above0(cl1) : R(0)};
-                if (wy && (FULL || i <= m.nx - 2))
+                if (wy && (FX || i <= m.nx - 2))
                     put_tile<1, MODE>(m, bf, boff, loff, soff, slo, top_xy, s0, cur, above);
             }
             // uz (base_solver.py:457-463, 511-517)
@@ -463,7 +466,7 @@ PF_DEV void tiled_march(const Mesh<R>& m, TiledLds<R, TJ, KZ>& L, int kl, int ro
                 V<R> s0 = leap_from<MODE, SPONGE>(cur, oin.o2, ol2, mget(smat, ks ? id_kp : c0.id, M_COEF), dv, s);
                 if (wall) s0 = {R(0), R(0)};
                 const V<R> above = {above0(c0.z), COMP ? above0(cl2) : R(0)};
-                if (wz && (FULL || i <= m.nx - 2))
+                if (wz && (FX || i <= m.nx - 2))
                     put_tile<2, MODE>(m, bf, boff, loff, soff, slo, top_z, s0, cur, above);
             }
         }
@@ -522,11 +525,14 @@ PF_DEV void tiled_tile(const Mesh<R>& m, const TileMap& tm, int t, TiledLds<R, T
     // block-uniform: the whole tile and chunk are clear of every boundary
     const int k_lo = kt * (KZ - 2) - 1, k_hi = k_lo + KZ - 1;
     const int j_lo = jt * jstep - 1, j_hi = j_lo + TJ - 1;
-    const bool full = k_lo >= 1 && k_hi <= m.nz - 4 && j_lo >= 1 && j_hi <= m.ny - 3 && ib >= 2 && ie <= m.nx - 3;
-    if (full)
-        tiled_march<R, MODE, SPONGE, TJ, KZ, true>(m, L, kl, row, k, j, ib, ie, w);
+    const bool fyz = k_lo >= 1 && k_hi <= m.nz - 4 && j_lo >= 1 && j_hi <= m.ny - 3;
+    const bool fx = ib >= 2 && ie <= m.nx - 3;
+    if (fyz && fx)
+        tiled_march<R, MODE, SPONGE, TJ, KZ, true, true>(m, L, kl, row, k, j, ib, ie, w);
+    else if (fyz)
+        tiled_march<R, MODE, SPONGE, TJ, KZ, true, false>(m, L, kl, row, k, j, ib, ie, w);
     else
-        tiled_march<R, MODE, SPONGE, TJ, KZ, false>(m, L, kl, row, k, j, ib, ie, w);
+        tiled_march<R, MODE, SPONGE, TJ, KZ, false, false>(m, L, kl, row, k, j, ib, ie, w);
 }
 
 // KZ lanes along z per tile row (64: one row per wavefront; 32: two rows
