@@ -203,12 +203,14 @@ int pf_dft(const double *in_re, const double *in_im, int64_t batch, int64_t n, i
            const double *window, double scale, double *out_re, double *out_im);
 
 /*
- * Timing: run nsteps steps with HIP events around every sweep and face
- * launch on the context's stream; returns the mean duration of each per
- * step (ms).  With PF_KERNEL_TILED2, sweep = the two-step pass and the
- * band (the level n+2 cells next to the x and y faces, which the pass
- * leaves) and face = the faces and probes of both steps, each divided by 2
- * per pair.
+ * Timing: run nsteps steps on the context's stream with HIP events around
+ * the sweep launch of every fourth step (with PF_KERNEL_TILED2, of every
+ * fourth pair of steps, whose sweep is the two-step pass), the first
+ * included, and one after the last step; returns the mean duration per
+ * step of the sweeps (sweep_ms) and of everything else (face_ms: faces,
+ * probes, launch gaps; with PF_KERNEL_TILED2 also the band, the level n+2
+ * cells next to the x and y faces that the two-step pass leaves).  Each
+ * event delays the launch after it by a few microseconds, hence the stride.
  */
 int pf_profile(pf_ctx *ctx, int64_t nsteps, double *sweep_ms, double *face_ms);
 

@@ -489,9 +489,13 @@ static int launch_sweep_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
 
 template <class R, int MODE>
 static int launch_faces_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
-    const long long n = face_total(c->nx, c->ny, c->nz);
-    const unsigned g = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL((k_faces<R, MODE>), dim3(g), dim3(256), 0, s, m, n);
+    long long rb, eb;
+    if (face_rows_grid(c->nx, c->ny, c->nz, rb, eb) && rb + eb <= 0x7fffffffLL) {
+        hipLaunchKernelGGL((k_faces_rows<R, MODE>), dim3((unsigned)(rb + eb)), dim3(256), 0, s, m, (int)eb);
+    } else {
+        const long long n = face_total(c->nx, c->ny, c->nz);
+        hipLaunchKernelGGL((k_faces<R, MODE>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m, n);
+    }
     HIPCHK(hipGetLastError());
     return PF_OK;
 }
@@ -514,7 +518,7 @@ static int launch_probe_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
 
 template <class R, int MODE>
 static int step_once_t(pf_ctx* c, hipStream_t s, const long long* base, long long off,
-                       hipEvent_t* ev /* 3 (of the 5 pf_profile gives) or null */) {
+                       hipEvent_t* ev /* 2 or null */) {
     Mesh<R> m = make_view<R>(c, c->cur, c->old, c->old);
     m.step_base = base;
     m.step_off = off;
@@ -523,7 +527,6 @@ static int step_once_t(pf_ctx* c, hipStream_t s, const long long* base, long lon
     if ((rc = launch_sweep_t<R, MODE>(c, m, s))) return rc;
     if (ev) HIPCHK(hipEventRecord(ev[1], s));
     if ((rc = launch_faces_t<R, MODE>(c, m, s))) return rc;
-    if (ev) HIPCHK(hipEventRecord(ev[2], s));
     if ((rc = launch_probe_t<R>(c, m, s))) return rc;
     const int t = c->cur;
     c->cur = c->old;
@@ -533,10 +536,13 @@ static int step_once_t(pf_ctx* c, hipStream_t s, const long long* base, long lon
 
 // Tile shape of the one-step tiled sweep by measurement, when the caller
 // leaves it open: each built shape sweeps the new context's own fields (all
-// zero at creation, step 0) a few times on its stream, and the fastest is
-// kept.  All shapes give the same bits, so only the speed depends on the
-// choice.  Only the sweep runs (no faces, probes or step counters), and the
-// level buffers are cleared afterwards.  The best shape varies with the
+// zero at creation, step 0) on its stream, and the fastest is kept.  All
+// shapes give the same bits, so only the speed depends on the choice.  Only
+// the sweep runs (no faces, probes or step counters), and the level buffers
+// are cleared afterwards.  Every shape first runs once untimed (the first
+// launch of a kernel and the first touch of the fields cost extra), then
+// the shapes are timed in two rounds, forward and backward, three sweeps
+// each, and each keeps its faster round.  The best shape varies with the
 // mesh (z extent against 62- or 30-lane tile rows, blocks per CU against
 // the CU count) more than a fixed rule follows.
 template <class R, int MODE>
@@ -548,36 +554,43 @@ static int tune_shape_t(pf_ctx* c) {
     const int nc = d64 ? 3 : 5;
     if (c->wave_len < 1) return PF_OK;  // the sweep would return at once
     hipEvent_t a = nullptr, b = nullptr;
-    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+    if (hipEventCreateWithFlags(&a, PF_TIMING_EVENT) != hipSuccess ||
+        hipEventCreateWithFlags(&b, PF_TIMING_EVENT) != hipSuccess) {
         if (a) (void)hipEventDestroy(a);
         (void)hipGetLastError();
         return fail(PF_ERR_HIP, "tile tuning: cannot create events");
     }
     const Mesh<R> m = make_view<R>(c, c->cur, c->old, c->old);
     const int j0 = c->tile_j, z0 = c->tile_z;
-    int bj = j0, bz = z0, rc = PF_OK;
-    float best = 0.f;
-    for (int s = 0; s < nc && !rc; s++) {
+    int rc = PF_OK;
+    float best[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int s = 0; s < nc && !rc; s++) {  // untimed: one sweep of every shape
         c->tile_j = cand[s][0];
         c->tile_z = cand[s][1];
-        float ms = 0.f;
-        if ((rc = launch_sweep_t<R, MODE>(c, m, c->stream))) break;  // warm-up
-        if (hipEventRecord(a, c->stream) != hipSuccess) { rc = fail(PF_ERR_HIP, "tile tuning: event"); break; }
-        for (int r = 0; r < 4 && !rc; r++) rc = launch_sweep_t<R, MODE>(c, m, c->stream);
-        if (rc) break;
-        if (hipEventRecord(b, c->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
-            hipEventElapsedTime(&ms, a, b) != hipSuccess) {
-            rc = fail(PF_ERR_HIP, "tile tuning: event timing");
-            break;
-        }
-        if (s == 0 || ms < best) {
-            best = ms;
-            bj = cand[s][0];
-            bz = cand[s][1];
+        rc = launch_sweep_t<R, MODE>(c, m, c->stream);
+    }
+    for (int round = 0; round < 2 && !rc; round++) {
+        for (int q = 0; q < nc && !rc; q++) {
+            const int s = round == 0 ? q : nc - 1 - q;
+            c->tile_j = cand[s][0];
+            c->tile_z = cand[s][1];
+            float ms = 0.f;
+            if (hipEventRecord(a, c->stream) != hipSuccess) { rc = fail(PF_ERR_HIP, "tile tuning: event"); break; }
+            for (int r = 0; r < 3 && !rc; r++) rc = launch_sweep_t<R, MODE>(c, m, c->stream);
+            if (rc) break;
+            if (hipEventRecord(b, c->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+                rc = fail(PF_ERR_HIP, "tile tuning: event timing");
+                break;
+            }
+            if (round == 0 || ms < best[s]) best[s] = ms;
         }
     }
-    c->tile_j = rc ? j0 : bj;
-    c->tile_z = rc ? z0 : bz;
+    int pick = 0;
+    for (int s = 1; s < nc; s++)
+        if (best[s] < best[pick]) pick = s;
+    c->tile_j = rc ? j0 : cand[pick][0];
+    c->tile_z = rc ? z0 : cand[pick][1];
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (rc) return rc;
@@ -617,7 +630,7 @@ static int launch_pair_s(const pf_ctx* c, const Mesh<R>& m, R* out2, hipStream_t
 
 // Steps tt, tt+1: the two-step pass writes level n+1 (view m1: n, n-1 ->
 // a1) and the non-band cells of level n+2 (a2); then the level n+1 faces
-// and probe frame, the band of level n+2 (view m2: n+1, n -> a2, its
+// and probe frame, the band of level n+2 (view m2: n+1, n -> a2), its
 // faces and probe frame.  a1, a2 are the two levels not in use.
 template <class R, int MODE>
 static int step_pair_t(pf_ctx* c, hipStream_t s, const long long* base, long long off, hipEvent_t* ev) {
@@ -637,7 +650,6 @@ static int step_pair_t(pf_ctx* c, hipStream_t s, const long long* base, long lon
     if (ev) HIPCHK(hipEventRecord(ev[1], s));
     if ((rc = launch_faces_t<R, MODE>(c, m1, s))) return rc;
     if ((rc = launch_probe_t<R>(c, m1, s))) return rc;
-    if (ev) HIPCHK(hipEventRecord(ev[2], s));
     const BandMap bm = band_map(c);
     const long long nb = band_blocks(bm);
     if (nb > 0x7fffffffLL) return fail(PF_ERR_ARG, "too many tiles");
@@ -646,10 +658,8 @@ static int step_pair_t(pf_ctx* c, hipStream_t s, const long long* base, long lon
     else
         hipLaunchKernelGGL((k_band<R, MODE, false>), dim3((unsigned)nb), dim3(64, 4), 0, s, m2, bm.j, bm.i);
     HIPCHK(hipGetLastError());
-    if (ev) HIPCHK(hipEventRecord(ev[3], s));
     if ((rc = launch_faces_t<R, MODE>(c, m2, s))) return rc;
     if ((rc = launch_probe_t<R>(c, m2, s))) return rc;
-    if (ev) HIPCHK(hipEventRecord(ev[4], s));
     c->old = a1;
     c->cur = a2;
     return PF_OK;
@@ -1019,42 +1029,50 @@ extern "C" int pf_profile(pf_ctx* c, int64_t nsteps, double* sweep_ms, double* f
     if (c->batch) return fail(PF_ERR_STATE, "context is stepped by its batch");
     if (nsteps < 1) return fail(PF_ERR_ARG, "nsteps < 1");
     if (c->step + nsteps > c->wave_len) return fail(PF_ERR_RANGE, "beyond the source table");
-    // events per launch group, a single step (0 | sweep | 1 | faces, probe
-    // | 2) or a pair of steps of the two-step sweep (0 | pass | 1 | level
-    // n+1 faces, probe | 2 | band of level n+2 | 3 | level n+2 faces,
-    // probe | 4).  Sweep time = the pass and the band (together they
-    // compute every non-face cell of both levels); the rest = faces and
-    // probes.
-    const int EV = 5;
-    std::vector<hipEvent_t> ev((size_t)nsteps * EV);
-    std::vector<char> pair((size_t)nsteps, 0);
-    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    // Launch groups: a pair's two-step pass with its band, faces and probes,
+    // or a single step.  Events bracket the sweep of every fourth group of
+    // each kind (the first included), and one event follows the last group:
+    // a recorded event is a marker between two launches that holds the next
+    // launch back by a few microseconds, so bracketing every group would
+    // slow the steps it times.  Sweep time = the mean bracketed sweep of
+    // each kind times the groups of that kind; the rest (faces, probes, the
+    // band of a pair, launch gaps) = the span from the first event to the
+    // last less the sweeps.
+    const int64_t stride = 4;
+    const int64_t pairs = c->kernel == PF_KERNEL_TILED2 ? nsteps / 2 : 0;
+    const int64_t singles = nsteps - 2 * pairs;
+    std::vector<hipEvent_t> ev((size_t)((pairs + stride - 1) / stride + (singles + stride - 1) / stride) * 2 + 1);
+    for (auto& e : ev) HIPCHK(hipEventCreateWithFlags(&e, PF_TIMING_EVENT));
     int rc = PF_OK;
-    int64_t groups = 0, s = 0;
-    if (c->kernel == PF_KERNEL_TILED2) {
-        for (; s + 2 <= nsteps && !rc; s += 2, groups++) {
-            rc = step_pair(c, c->stream, nullptr, c->step, &ev[(size_t)groups * EV]);
-            pair[(size_t)groups] = 1;
-            if (!rc) c->step += 2;
-        }
+    size_t used = 0;
+    for (int64_t g = 0; g < pairs && !rc; g++) {
+        hipEvent_t* e = g % stride == 0 ? &ev[used] : nullptr;
+        if (e) used += 2;
+        rc = step_pair(c, c->stream, nullptr, c->step, e);
+        if (!rc) c->step += 2;
     }
-    for (; s < nsteps && !rc; s++, groups++) {
-        rc = step_once(c, c->stream, nullptr, c->step, &ev[(size_t)groups * EV]);
+    const size_t pair_ev = used;
+    for (int64_t g = 0; g < singles && !rc; g++) {
+        hipEvent_t* e = g % stride == 0 ? &ev[used] : nullptr;
+        if (e) used += 2;
+        rc = step_once(c, c->stream, nullptr, c->step, e);
         if (!rc) c->step++;
     }
     if (!rc) {
+        HIPCHK(hipEventRecord(ev[used], c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
-        double sw = 0, fc = 0;
-        for (int64_t g = 0; g < groups; g++) {
-            const hipEvent_t* e = &ev[(size_t)g * EV];
-            float t[4] = {0, 0, 0, 0};
-            const int n = pair[(size_t)g] ? 4 : 2;
-            for (int k = 0; k < n; k++) HIPCHK(hipEventElapsedTime(&t[k], e[k], e[k + 1]));
-            sw += t[0] + t[2];
-            fc += t[1] + t[3];
+        double sum[2] = {0, 0};
+        for (size_t g = 0; g < used; g += 2) {
+            float t = 0;
+            HIPCHK(hipEventElapsedTime(&t, ev[g], ev[g + 1]));
+            sum[g < pair_ev ? 0 : 1] += t;
         }
-        *sweep_ms = sw / nsteps;
-        *face_ms = fc / nsteps;
+        const double np = (double)(pair_ev / 2), ns = (double)((used - pair_ev) / 2);
+        const double sw = (np > 0 ? sum[0] / np * (double)pairs : 0.0) + (ns > 0 ? sum[1] / ns * (double)singles : 0.0);
+        float all = 0;
+        HIPCHK(hipEventElapsedTime(&all, ev[0], ev[used]));
+        *sweep_ms = sw / (double)nsteps;
+        *face_ms = ((double)all - sw) / (double)nsteps;
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     return rc;

@@ -578,8 +578,8 @@ extern "C" int pf_batch_autotune(pf_batch* b, int32_t trial_steps, int32_t* tile
     double best = 1e300;
     Cand bc = cands[0];
     hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventCreateWithFlags(&e0, PF_TIMING_EVENT));
+    HIPCHK(hipEventCreateWithFlags(&e1, PF_TIMING_EVENT));
     int rc = PF_OK;
     for (const Cand& k : cands) {
         b->tile_j = k.tj; b->tile_z = k.tz; b->tile_x = k.tx;

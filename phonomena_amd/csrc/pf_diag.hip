@@ -114,8 +114,9 @@ extern "C" int pf_stream_probe(int32_t reads, int32_t writes, int64_t n, int32_t
         if (hipMemset(p, 0, (size_t)n * sizeof(float)) != hipSuccess) { cleanup(); return fail(PF_ERR_HIP, "memset"); }
         if (i < reads) s.in[i] = (const f4*)p; else s.out[i - reads] = (f4*)p;
     }
-    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&a) != hipSuccess ||
-        hipEventCreate(&b) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&a, PF_TIMING_EVENT) != hipSuccess ||
+        hipEventCreateWithFlags(&b, PF_TIMING_EVENT) != hipSuccess) {
         cleanup();
         return fail(PF_ERR_HIP, "stream probe: device set-up failed");
     }

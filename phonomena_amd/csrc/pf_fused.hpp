@@ -498,12 +498,14 @@ __global__ __launch_bounds__(TJ * KZ, 1) void k_sweep_fused(Mesh<R> m, R* out2, 
 //   I: i in {nx-3, nx-2}, 2 <= j <= ny-4, k <= nz-2
 // (no component has a non-face cell at i = nx-1, j = 0, j = ny-1 or
 // k = nz-1; the z = max faces above k = nz-2 / nz-3 come with their cell).
-// The one-step tiled march sweeps them on tiles of 4 rows x 64 lanes:
-//   J tiles: two tile rows, rows -1..2 (writer rows 0 and 1; row 0 has no
-//            non-face cell) and rows ny-4..ny-1 (writer rows ny-3, ny-2),
-//            over all planes in chunks of 8;
-//   I tiles: every tile row (stride 2), over the one chunk of the planes
-//            nx-3, nx-2.
+// The one-step tiled march sweeps them:
+//   J tiles, 4 rows x 64 lanes: two tile rows, rows -1..2 (writer rows 0
+//            and 1; row 0 has no non-face cell) and rows ny-4..ny-1
+//            (writer rows ny-3, ny-2), over all planes in chunks of 8;
+//   I tiles, 8 rows x 32 lanes: every tile row (stride 6, writer rows
+//            6 jt .. 6 jt + 5), over the one chunk of the planes nx-3,
+//            nx-2 (a chunk of two planes reads four, so taller tiles,
+//            which reread fewer halo rows, pay more than longer rows).
 // So every tile stores band cells only; the J cells of planes nx-3, nx-2
 // are stored by a J and an I tile, from the same inputs with the same
 // code, so with the same bits.  Needs nx, ny, nz >= 8.
@@ -515,7 +517,7 @@ inline BandMap band_map(const pf_ctx* c) {
     BandMap b;
     b.j = tile_map(c, 4, 64, 8);
     b.j.njt = 2;
-    b.i = tile_map(c, 4, 64, 2);
+    b.i = tile_map(c, 8, 32, 2);
     b.i.nxc = 1;
     return b;
 }
@@ -526,11 +528,16 @@ inline long long band_blocks(const BandMap& b) {
 
 template <class R, int MODE, bool SPONGE>
 __global__ __launch_bounds__(256, (MODE == MODE_FP32 ? 4 : 2)) void k_band(Mesh<R> m, TileMap tj, TileMap ti) {
-    __shared__ TiledLds<R, 4, 64> L;
+    __shared__ union {
+        TiledLds<R, 4, 64> j;
+        TiledLds<R, 8, 32> i;
+    } L;
     const int nj = tj.nkt * tj.njt * tj.nxc;
     const int b = (int)blockIdx.x;
-    const bool jb = b < nj;
-    tiled_tile<R, MODE, SPONGE, 4, 64>(m, jb ? tj : ti, jb ? b : b - nj, L, jb ? m.ny - 3 : 2, jb ? 0 : m.nx - 3);
+    if (b < nj)
+        tiled_tile<R, MODE, SPONGE, 4, 64>(m, tj, b, L.j, m.ny - 3, 0);
+    else
+        tiled_tile<R, MODE, SPONGE, 8, 32>(m, ti, b - nj, L.i, 6, m.nx - 3);
 }
 
 }  // namespace pf

@@ -400,6 +400,122 @@ __global__ __launch_bounds__(256) void k_faces(Mesh<R> m, long long total) {
     faces_run<R, MODE, 1>(m, (long long)blockIdx.x * 256, total);
 }
 
+// The same face cells, arranged for one mesh's step.  The rows of the three
+// face planes
+//   X  (Lx, j, k)  0 < j < Ly,   Y0 (i, 0, k), Y1 (i, Ly, k)  i < Lx,   k < Lz
+// take one Mur map of interior cells each:
+//   X  map(cur(Lx-1, j, k), new(Lx-1, j, k), cur(Lx, j, k))
+//   Y0 map(cur(i, 1, k),    new(i, 1, k),    cur(i, 0, k))
+//   Y1 map(cur(i, Ly-1, k), new(i, Ly-1, k), cur(i, Ly, k))
+// (final_value's chain, with after_x and after_y0 reduced to the stage-I
+// value new(), which holds for i < Lx and Ly - 1 > 0).  A wave takes a run
+// of 256 cells of one row, four a lane, and issues all its loads before it
+// stores anything, so one round of memory latency covers four cells.  The
+// box edges (the rest of face_cell's list, a few thousand cells) go one a
+// thread through final_value, in the first blocks (their longer chains
+// start first).  Needs Ly >= 2 for every component.
+PF_DEV int face_runs(int nz) { return (nz - 1 + 255) / 256; }
+
+template <int Q>
+PF_DEV int face_rows(int nx, int ny) {
+    return Ext<Q>::Ly(ny) - 1 + 2 * Ext<Q>::Lx(nx);
+}
+
+template <int Q, int MODE, class R>
+PF_DEV void face_row_run(const Mesh<R>& m, int r, int run, int lane, R w) {
+    const int Lx = Ext<Q>::Lx(m.nx), Ly = Ext<Q>::Ly(m.ny), Lz = Ext<Q>::Lz(m.nz);
+    int i, j, ii, ji, f;  // the face row, the interior row its map reads, the face index
+    if (r < Ly - 1) {
+        i = Lx; j = 1 + r; ii = Lx - 1; ji = j; f = 0;
+    } else if (r < Ly - 1 + Lx) {
+        i = r - (Ly - 1); j = 0; ii = i; ji = 1; f = 3;
+    } else {
+        i = r - (Ly - 1) - Lx; j = Ly; ii = i; ji = Ly - 1; f = 6;
+    }
+    const R* c = Q == 0 ? m.c0 : (Q == 1 ? m.c1 : m.c2);
+    const R* n = Q == 0 ? m.n0 : (Q == 1 ? m.n1 : m.n2);
+    const lo_t* cl = Q == 0 ? m.cl0 : (Q == 1 ? m.cl1 : m.cl2);
+    const lo_t* nl = Q == 0 ? m.nl0 : (Q == 1 ? m.nl1 : m.nl2);
+    const long long fb = at(m, i, j, 0), ib = at(m, ii, ji, 0);
+    V<R> a[4], b[4], d[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int k = run * 256 + u * 64 + lane;
+        const int kk = k < Lz ? k : Lz - 1;  // lanes past the row load its last cell and store nothing
+        a[u].h = c[ib + kk];
+        b[u].h = n[ib + kk];
+        d[u].h = c[fb + kk];
+        if constexpr (MODE == MODE_FP32C) {
+            a[u].l = lo_decode(a[u].h, cl[ib + kk]);
+            b[u].l = lo_decode(b[u].h, nl[ib + kk]);
+            d[u].l = lo_decode(d[u].h, cl[fb + kk]);
+        } else {
+            a[u].l = b[u].l = d[u].l = R(0);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int k = run * 256 + u * 64 + lane;
+        if (k >= Lz) continue;
+        if (Q == 2 && k == 0) {  // the source line uz[0, :, 0] reads as w
+            if (ii == 0) { a[u].h = w; a[u].l = R(0); }
+            if (i == 0) { d[u].h = w; d[u].l = R(0); }
+        }
+        put<Q, MODE>(m, i, j, k, mur_map<MODE>(a[u], m.mur[f + Q], m.mur_lo[f + Q], b[u], d[u]));
+    }
+}
+
+template <int Q>
+PF_DEV long long face_edge_base(int nx, int ny, int nz) {
+    const long long Lx = Ext<Q>::Lx(nx), Ly = Ext<Q>::Ly(ny), Lz = Ext<Q>::Lz(nz);
+    return (Ly - 1) * Lz + 2 * Lx * Lz;
+}
+
+template <class R, int MODE>
+__global__ __launch_bounds__(256) void k_faces_rows(Mesh<R> m, int edge_blocks) {
+    const long long tt = step_index(m);
+    if (tt >= m.nsteps) return;
+    const R w = (R)m.wave[tt];
+    if ((int)blockIdx.x >= edge_blocks) {
+        const int kr = face_runs(m.nz);
+        const int lane = threadIdx.x % 64;
+        int t = ((int)blockIdx.x - edge_blocks) * 4 + (int)threadIdx.x / 64;  // this wave's run
+        const int w0 = face_rows<0>(m.nx, m.ny) * kr;
+        if (t < w0) { face_row_run<0, MODE>(m, t / kr, t % kr, lane, w); return; }
+        t -= w0;
+        const int w1 = face_rows<1>(m.nx, m.ny) * kr;
+        if (t < w1) { face_row_run<1, MODE>(m, t / kr, t % kr, lane, w); return; }
+        t -= w1;
+        if (t < face_rows<2>(m.nx, m.ny) * kr) face_row_run<2, MODE>(m, t / kr, t % kr, lane, w);
+        return;
+    }
+    long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long e0 = face_count<0>(m.nx, m.ny, m.nz) - face_edge_base<0>(m.nx, m.ny, m.nz);
+    if (e < e0) { face_write<0, MODE>(m, face_edge_base<0>(m.nx, m.ny, m.nz) + e, w); return; }
+    e -= e0;
+    const long long e1 = face_count<1>(m.nx, m.ny, m.nz) - face_edge_base<1>(m.nx, m.ny, m.nz);
+    if (e < e1) { face_write<1, MODE>(m, face_edge_base<1>(m.nx, m.ny, m.nz) + e, w); return; }
+    e -= e1;
+    const long long e2 = face_count<2>(m.nx, m.ny, m.nz) - face_edge_base<2>(m.nx, m.ny, m.nz);
+    if (e < e2) face_write<2, MODE>(m, face_edge_base<2>(m.nx, m.ny, m.nz) + e, w);
+}
+
+// Blocks of k_faces_rows for an nx x ny x nz mesh: edge_blocks, then
+// row_blocks (four runs each); false when the mesh is too small for it.
+inline bool face_rows_grid(int nx, int ny, int nz, long long& row_blocks, long long& edge_blocks) {
+    if (nx < 3 || ny < 4 || nz < 3) return false;
+    const long long kr = (nz - 1 + 255) / 256;
+    long long runs = 0, edges = 0;
+    const long long L[3][3] = {{nx - 2, ny - 1, nz - 1}, {nx - 1, ny - 2, nz - 1}, {nx - 1, ny - 1, nz - 2}};
+    for (const auto& e : L) {
+        runs += (e[1] - 1 + 2 * e[0]) * kr;
+        edges += 2 * (e[2] + 1) + (e[1] - 1) + 2 * e[0];
+    }
+    row_blocks = (runs + 3) / 4;
+    edge_blocks = (edges + 255) / 256;
+    return true;
+}
+
 inline long long face_total(int nx, int ny, int nz) {
     auto cnt = [](long long Lx, long long Ly, long long Lz) {
         return (Ly - 1) * Lz + 2 * Lx * Lz + 2 * (Lz + 1) + (Ly - 1) + 2 * Lx;

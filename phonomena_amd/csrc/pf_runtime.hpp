@@ -5,6 +5,13 @@
 #include "../../include/phonofdtd.h"
 #include "pf_kernels.hpp"
 
+// Flags of the events the library times launches with: no system-scope
+// fence when an event is recorded (a recorded event would otherwise write
+// back and invalidate the caches between the launches it brackets, which
+// slows the launches after it; the elapsed times are read only after a
+// stream synchronization, which makes the event visible).
+#define PF_TIMING_EVENT hipEventDisableSystemFence
+
 // Material rows the tiled sweep stages in LDS.
 #define PF_TILED_MAX_MATERIALS 16
 // Snapshot slots a context can hold.
@@ -90,10 +97,11 @@ void ctx_free(pf_ctx* c);
 template <class R>
 Mesh<R> make_view(const pf_ctx* c, int cur, int old, int out);
 // Issue one step (sweep, faces, probe) on stream s; tt = *base + off (or
-// off when base is null).  ev: 3 events recorded around the launches, or null.
+// off when base is null).  ev: 2 events recorded before and after the
+// sweep, or null.
 int step_once(pf_ctx* c, hipStream_t s, const long long* base, long long off, hipEvent_t* ev);
 // Issue steps tt and tt+1 with the two-step sweep (PF_KERNEL_TILED2); ev as
-// for step_once (the middle event follows the two-step pass).
+// for step_once (around the two-step pass).
 int step_pair(pf_ctx* c, hipStream_t s, const long long* base, long long off, hipEvent_t* ev);
 // Device address of the allocation-free staging array on device dev, taken
 // once per device (called at context creation, with dev current).

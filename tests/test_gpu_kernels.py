@@ -141,3 +141,41 @@ def test_unmeasured_context_takes_the_fixed_shape(precision):
         assert ctx.kernel_info() == ("tiled",) + fdtd.STATIC_TILES[precision]
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("precision,kernel,tiles", [("fp64", "tiled", (8, 32)), ("fp32", "tiled", (8, 64)),
+                                                     ("fp32c", "tiled", (32, 32)), ("fp32", "tiled2", (32, 32)),
+                                                     ("fp32c", "tiled2", (16, 64))])
+def test_tiled_and_pair_match_naive_with_layer_on_larger_mesh(precision, kernel, tiles):
+    # a mesh large enough for boundary-free tiles both inside and across the
+    # absorbing layer, from a random start, so every tile kind computes
+    # non-trivial values: the tiled sweeps give the naive sweep's bits
+    x, y, z, ids, rows, dt, wave = uniform_crystal(
+        (150, 120, 150), 1.0, incl=(19300.0, 2.0e11, 1.6e11, 18, 6), steps=30, seed=8, wave_kind="gauss",
+        f=1.0 / (12 * 0.5 / 6176.0))
+    nodes = (x.size, y.size, z.size)
+    sponge = problem.sponge_tables(nodes, {"width": 10, "strength": 0.15})
+    rng = np.random.default_rng(3)
+    spec = spec_from(x, y, z, ids, rows, dt, wave, probe=(0, 4, 36, 0, 4, 30), every=5, sponge=sponge)
+    start = {c: rng.standard_normal(s) for c, s in (("ux", (nodes[0] - 1,) + nodes[1:]),
+                                                   ("uy", (nodes[0], nodes[1] - 1, nodes[2])),
+                                                   ("uz", nodes[:2] + (nodes[2] - 1,)))}
+    for c in ("uy", "uz"):
+        start[c][0] = 0.0  # the x = 0 wall stays 0 in the reference
+
+    def run(kern, tj=0, tz=0):
+        ctx = fdtd.Context(spec, precision=precision, kernel=kern, tile_j=tj, tile_z=tz)
+        for c in ("ux", "uy", "uz"):
+            ctx.set(c, start[c])
+            ctx.set(c + "_old", 0.5 * start[c])
+        ctx.step(30)
+        ctx.sync()
+        out = {f: ctx.get(f) for f in FIELDS}
+        out["probes"] = ctx.probes()
+        ctx.close()
+        return out
+
+    ref = run("naive")
+    got = run(kernel, *tiles)
+    for f in ref:
+        assert np.array_equal(ref[f], got[f]), (precision, kernel, f)
