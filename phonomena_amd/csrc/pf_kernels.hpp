@@ -409,12 +409,16 @@ __global__ __launch_bounds__(256) void k_faces(Mesh<R> m, long long total) {
 //   Y1 map(cur(i, Ly-1, k), new(i, Ly-1, k), cur(i, Ly, k))
 // (final_value's chain, with after_x and after_y0 reduced to the stage-I
 // value new(), which holds for i < Lx and Ly - 1 > 0).  A wave takes a run
-// of 256 cells of one row, four a lane, and issues all its loads before it
-// stores anything, so one round of memory latency covers four cells.  The
+// of FACE_RUN cells of one row, FACE_LANE_CELLS a lane, and issues all its
+// loads before it stores anything, so one round of memory latency covers
+// them (eight a lane measured no faster than four: the launch is bound by
+// its fixed cost, about 4 us of a 9.6 us launch at 513 x 513 x 257).  The
 // box edges (the rest of face_cell's list, a few thousand cells) go one a
 // thread through final_value, in the first blocks (their longer chains
 // start first).  Needs Ly >= 2 for every component.
-PF_DEV int face_runs(int nz) { return (nz - 1 + 255) / 256; }
+constexpr int FACE_LANE_CELLS = 4;                 // cells of a row per lane
+constexpr int FACE_RUN = 64 * FACE_LANE_CELLS;     // cells of a row per wave
+PF_DEV int face_runs(int nz) { return (nz - 1 + FACE_RUN - 1) / FACE_RUN; }
 
 template <int Q>
 PF_DEV int face_rows(int nx, int ny) {
@@ -437,10 +441,10 @@ PF_DEV void face_row_run(const Mesh<R>& m, int r, int run, int lane, R w) {
     const lo_t* cl = Q == 0 ? m.cl0 : (Q == 1 ? m.cl1 : m.cl2);
     const lo_t* nl = Q == 0 ? m.nl0 : (Q == 1 ? m.nl1 : m.nl2);
     const long long fb = at(m, i, j, 0), ib = at(m, ii, ji, 0);
-    V<R> a[4], b[4], d[4];
+    V<R> a[FACE_LANE_CELLS], b[FACE_LANE_CELLS], d[FACE_LANE_CELLS];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-        const int k = run * 256 + u * 64 + lane;
+    for (int u = 0; u < FACE_LANE_CELLS; u++) {
+        const int k = run * FACE_RUN + u * 64 + lane;
         const int kk = k < Lz ? k : Lz - 1;  // lanes past the row load its last cell and store nothing
         a[u].h = c[ib + kk];
         b[u].h = n[ib + kk];
@@ -454,8 +458,8 @@ PF_DEV void face_row_run(const Mesh<R>& m, int r, int run, int lane, R w) {
         }
     }
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-        const int k = run * 256 + u * 64 + lane;
+    for (int u = 0; u < FACE_LANE_CELLS; u++) {
+        const int k = run * FACE_RUN + u * 64 + lane;
         if (k >= Lz) continue;
         if (Q == 2 && k == 0) {  // the source line uz[0, :, 0] reads as w
             if (ii == 0) { a[u].h = w; a[u].l = R(0); }
@@ -504,7 +508,7 @@ __global__ __launch_bounds__(256) void k_faces_rows(Mesh<R> m, int edge_blocks) 
 // row_blocks (four runs each); false when the mesh is too small for it.
 inline bool face_rows_grid(int nx, int ny, int nz, long long& row_blocks, long long& edge_blocks) {
     if (nx < 3 || ny < 4 || nz < 3) return false;
-    const long long kr = (nz - 1 + 255) / 256;
+    const long long kr = (nz - 1 + FACE_RUN - 1) / FACE_RUN;
     long long runs = 0, edges = 0;
     const long long L[3][3] = {{nx - 2, ny - 1, nz - 1}, {nx - 1, ny - 2, nz - 1}, {nx - 1, ny - 1, nz - 2}};
     for (const auto& e : L) {
