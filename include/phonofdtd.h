@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define PF_ABI_VERSION 1
+#define PF_ABI_VERSION 2
 
 #define PF_OK 0
 #define PF_ERR_ARG (-1)     /* invalid argument or shape                 */
@@ -121,6 +121,17 @@ typedef struct pf_desc {
                                   float32 target record, widened), K_m = z
                                   nodes covered (0..nz, from the reference's
                                   z test)                                      */
+    /* Free surface (k = 0).  0: the reference's, bit for bit, including its
+       precedence in the u_z update (base_solver.py:514-516), where T3[..,1]
+       enters unscaled and only T3[..,0] (= 0) is divided by fdz.  1: no
+       counterpart in the reference, for physical spacings: the difference
+       is divided, (T3[..,1] - T3[..,0]) / fdz, as in the interior update
+       (:461-463).  With fdz = 1 both give the same bits; on a uniform grid
+       the fixed scheme's fields do not depend on the spacing (with dt
+       scaled alike), the reference's diverge at physical spacings.  The
+       surface spacings stay the reference's (its first-only and swapped
+       spacings at k = 0 are exact no-ops on uniform grids). */
+    int32_t surface;
 } pf_desc;
 
 /* Library / device identification. */

@@ -263,11 +263,14 @@ def displacement(s, P, dt, sp, damp=None):
                          _sl(damp, "z", sl))
 
 
-def displacement_surface(s, P, dt, sp, damp=None):
+def displacement_surface(s, P, dt, sp, damp=None, fixed=False):
     """Displacements on the free surface k=0, base_solver.py:495-517.
 
     The u_z line keeps the reference's operator precedence: T3[...,1] enters
-    unscaled and only T3[...,0] (always 0) is divided by fdz.
+    unscaled and only T3[...,0] (always 0) is divided by fdz.  ``fixed``
+    (extension, not the reference) divides the difference instead,
+    (T3[...,1] - T3[...,0]) / fdz, as the interior u_z update does
+    (base_solver.py:461-463); with fdz = 1 both give the same bits.
     """
     D = dt ** 2
     sx0 = sp["sdx"][0, :, :]
@@ -292,7 +295,8 @@ def displacement_surface(s, P, dt, sp, damp=None):
         s.uz[1:-1, 1:-1, 0], s.uz_old[1:-1, 1:-1, 0], D / P[1:-1, 1:-1, 0],
         (s.T5[1:, 1:-1, 0] - s.T5[:-1, 1:-1, 0]) / sx0
         + (s.T4[1:-1, 1:, 0] - s.T4[1:-1, :-1, 0]) / sy0
-        + s.T3[1:-1, 1:-1, 1] - s.T3[1:-1, 1:-1, 0] / fz0,
+        + ((s.T3[1:-1, 1:-1, 1] - s.T3[1:-1, 1:-1, 0]) / fz0 if fixed
+           else s.T3[1:-1, 1:-1, 1] - s.T3[1:-1, 1:-1, 0] / fz0),
         _sl(damp, "z", (slice(1, -1), slice(1, -1), 0)))
 
 
@@ -339,8 +343,9 @@ def sponge_fields(shape, prof):
 class Problem:
     """Everything one run needs, in the reference's layouts."""
 
-    def __init__(self, x, y, z, C, P, dt, wave, mur_c=None, sponge=None, si=1.0):
+    def __init__(self, x, y, z, C, P, dt, wave, mur_c=None, sponge=None, si=1.0, surface_fixed=False):
         self.x, self.y, self.z = x, y, z
+        self.surface_fixed = surface_fixed
         self.sp = spacings(x, y, z, si)
         self.C, self.P, self.dt = C, P, dt
         self.wave = np.asarray(wave, dtype=np.float64)
@@ -373,7 +378,7 @@ def step(s, pb, tt):
     stress(s, pb.C, pb.sp)
     stress_surface(s, pb.C, pb.sp)
     displacement(s, pb.P, pb.dt, pb.sp, pb.damp)
-    displacement_surface(s, pb.P, pb.dt, pb.sp, pb.damp)
+    displacement_surface(s, pb.P, pb.dt, pb.sp, pb.damp, pb.surface_fixed)
     mur(s, pb.mur_c)
     rotate(s)
 

@@ -100,6 +100,7 @@ int check_desc(const pf_desc* d) {
         }
     }
     if (!d->mur) return fail(PF_ERR_ARG, "Mur coefficients missing");
+    if (d->surface != 0 && d->surface != 1) return fail(PF_ERR_ARG, "surface must be 0 or 1 (got %d)", d->surface);
     if (!d->wave || d->wave_len < 0) return fail(PF_ERR_ARG, "source table missing");
     if (!(d->dt > 0) || !(d->dt2 > 0)) return fail(PF_ERR_ARG, "dt must be > 0");
     const bool any = d->sponge_x || d->sponge_y || d->sponge_z;
@@ -299,6 +300,7 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
     }
     c->esz = d->precision == PF_FP64 ? 8 : 4;
     c->sponge = d->sponge_x != nullptr;
+    c->surface = d->surface;
     c->dt = d->dt;
     c->nmat = d->n_materials;
     HIPCHK(hipGetDevice(&c->dev));
@@ -468,6 +470,7 @@ Mesh<R> pf::make_view(const pf_ctx* c, int cur, int old, int nxt) {
     m.step_off = 0;
     m.nsteps = c->wave_len;
     m.nmat = c->nmat;
+    m.surf = c->surface;
     return m;
 }
 

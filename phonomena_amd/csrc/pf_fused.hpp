@@ -120,7 +120,7 @@ PF_DEV void fused_march(const Mesh<R>& m, R* out2, FusedLds<R, TJ, KZ>& L, int k
     const R s5_h1 = sel(ks, hfz, m.fx0);
     const R s6_h1 = sel(ks, hfy, m.fx0);
     const R uy_g2 = sel(ks, hfy, m.fy0);
-    const R uz_g3 = sel(ks, hfz, R(1));
+    const R uz_g3 = sel(ks || m.surf, hfz, R(1));  // as in the one-step sweep
     R dpyz = R(0), dpz_k = R(0);
     if constexpr (SPONGE) {
         dpyz = m.dpy[j < 0 ? 0 : (j < m.ny ? j : m.ny - 1)];

@@ -78,6 +78,7 @@ struct Mesh {
     long long step_off;                       // tt = *step_base + step_off
     long long nsteps;                         // step budget (batch members)
     int nmat;                                 // rows in mat
+    int surf;  // 1: the surface u_z update divides T3's difference by fdz (extension); 0: the reference's precedence
 };
 
 template <class R>
@@ -217,7 +218,8 @@ PF_DEV void stage_uz(const Mesh<R>& m, int i, int j, int k, R w, R& coef, R& dv)
         dv = div3<EXACT>(d5, m.sx[i - 1], d4, m.sy[j - 1], a3 - b3, m.fz[k]);
         coef = matp(m, i, j, k + 1)[M_COEF];
     } else {
-        dv = div3_uz_surface<EXACT>(d5, m.sx0, d4, m.sy0, a3, b3, m.fz0);
+        dv = m.surf ? div3<EXACT>(d5, m.sx0, d4, m.sy0, a3 - b3, m.fz0)
+                    : div3_uz_surface<EXACT>(d5, m.sx0, d4, m.sy0, a3, b3, m.fz0);
         coef = matp(m, i, j, 0)[M_COEF];  // reference samples rho at k = 0 here
     }
 }

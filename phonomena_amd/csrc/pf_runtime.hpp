@@ -32,6 +32,7 @@ struct pf_ctx {
     bool shape_open = false;     // tile shape left to measurement at creation
     size_t esz = 8;
     bool sponge = false;
+    int surface = 0;  // pf_desc.surface
     double dt = 0;
     int nmat = 0;
     int dev = 0;

@@ -246,7 +246,10 @@ __global__ __launch_bounds__(64 * W, (MODE == MODE_FP32 ? 4 : 2)) void k_sweep_s
                         dv = div3<EXACT>(s5[rr] - s5m[rr], hsx, s4[rr] - s4[rr - 1], hsy[rr], n3_kp - n3_0[rr], hfz);
                         coef = smat[MAT_STRIDE * d0_kp + M_COEF];
                     } else {
-                        dv = div3_uz_surface<EXACT>(s5[rr] - s5m[rr], m.sx0, s4[rr] - s4[rr - 1], m.sy0, n3_kp, n3_0[rr], m.fz0);
+                        dv = m.surf ? div3<EXACT>(s5[rr] - s5m[rr], m.sx0, s4[rr] - s4[rr - 1], m.sy0,
+                                                  n3_kp - n3_0[rr], m.fz0)
+                                    : div3_uz_surface<EXACT>(s5[rr] - s5m[rr], m.sx0, s4[rr] - s4[rr - 1], m.sy0,
+                                                             n3_kp, n3_0[rr], m.fz0);
                         coef = smat[MAT_STRIDE * d0[rr] + M_COEF];
                     }
                     s0 = leap_from<MODE, SPONGE>(cur, o2[rr], ol2[rr], coef, dv, s);

@@ -314,7 +314,9 @@ PF_DEV void tiled_march(const Mesh<R>& m, TiledLds<R, TJ, KZ>& L, int kl, int ro
     const R s5_h1 = sel(ks, hfz, m.fx0);
     const R s6_h1 = sel(ks, hfy, m.fx0);
     const R uy_g2 = sel(ks, hfy, m.fy0);
-    const R uz_g3 = sel(ks, hfz, R(1));
+    // u_z's T3 term at k = 0: the reference's unscaled T3[..,1], or (surf)
+    // the difference divided by fdz, as in the interior
+    const R uz_g3 = sel(ks || m.surf, hfz, R(1));
     R dpyz = R(0), dpz_k = R(0);
     if constexpr (SPONGE) {
         dpyz = m.dpy[j < 0 ? 0 : (j < m.ny ? j : m.ny - 1)];

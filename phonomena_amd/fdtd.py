@@ -51,6 +51,7 @@ class PfDesc(ctypes.Structure):
         ("tile_j", ctypes.c_int32), ("tile_x", ctypes.c_int32), ("tile_z", ctypes.c_int32),
         ("node_x", ctypes.c_void_p), ("node_y", ctypes.c_void_p),
         ("n_targets", ctypes.c_int32), ("targets", ctypes.c_void_p),
+        ("surface", ctypes.c_int32),
     ]
 
 
@@ -108,8 +109,8 @@ def load(path=None):
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = res
-    if lib.pf_abi_version() != 1:
-        raise FdtdUnavailable("libphonofdtd ABI %d, expected 1" % lib.pf_abi_version())
+    if lib.pf_abi_version() != 2:
+        raise FdtdUnavailable("libphonofdtd ABI %d, expected 2" % lib.pf_abi_version())
     if path is None:
         _lib = lib
     return lib
@@ -152,11 +153,13 @@ class MeshSpec:
     (i0, si, ni, j0, sj, nj) or None.  With ids None, raster = (x, y,
     targets) gives the node coordinates along x and y and the (n, 4)
     inclusion list (tx, ty, r, z nodes covered) the device rasterizes the
-    material map from (problem.raster_targets).
+    material map from (problem.raster_targets).  surface: 0 for the
+    reference's free surface (bit for bit), 1 for the one with the u_z
+    update's precedence fixed (pf_desc.surface).
     """
 
     def __init__(self, nodes, fd, sd, materials, ids, dt, dt2, mur, wave,
-                 sponge=None, probe=None, record_every=1, probe_capacity=1, raster=None):
+                 sponge=None, probe=None, record_every=1, probe_capacity=1, raster=None, surface=0):
         self.nodes = tuple(int(n) for n in nodes)
         self.fd = [_c(a, np.float64) for a in fd]
         self.sd = [_c(a, np.float64) for a in sd]
@@ -176,6 +179,9 @@ class MeshSpec:
         self.probe = None if probe is None else tuple(int(v) for v in probe)
         self.record_every = int(record_every)
         self.probe_capacity = int(probe_capacity)
+        if surface not in (0, 1):
+            raise ValueError("surface must be 0 (the reference's) or 1 (precedence fixed)")
+        self.surface = int(surface)
         nx, ny, nz = self.nodes
         if self.raster is not None:
             if self.raster[0].shape != (nx,) or self.raster[1].shape != (ny,):
@@ -243,6 +249,7 @@ class Context:
             d.record_every = spec.record_every
             d.probe_capacity = spec.probe_capacity
         d.tile_j, d.tile_x, d.tile_z = int(tile_j), int(tile_x), int(tile_z)
+        d.surface = getattr(spec, "surface", 0)
         h = ctypes.c_void_p()
         check(self.lib.pf_create(ctypes.byref(d), ctypes.byref(h)))
         self.handle = h

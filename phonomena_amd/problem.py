@@ -220,7 +220,10 @@ def mesh_spec(g, m, wave, cfg, device_ids=False):
     inclusion list (m.ids is not needed).
 
     ``m`` is this package's Material or the reference's (dense ``C``/``P``,
-    material.py:22-93): see material_table."""
+    material.py:22-93): see material_table.  ``cfg["free_surface"]``:
+    "reference" (default; the reference's free surface, bit for bit) or
+    "fixed" (its u_z precedence corrected, for physical spacings; no
+    counterpart in the reference)."""
     nodes = (g.x.size, g.y.size, g.z.size)
     fd, sd = axis_spacings(g)
     dt = m.dt
@@ -241,4 +244,16 @@ def mesh_spec(g, m, wave, cfg, device_ids=False):
     cap = int(cfg.get("probe_capacity", 0)) or max(1, steps // max(every, 1))
     return MeshSpec(nodes, fd, sd, rows, ids, dt, dt2, mur, wave,
                     sponge=sponge_tables(nodes, cfg.get("sponge")),
-                    probe=probe, record_every=every, probe_capacity=cap, raster=raster)
+                    probe=probe, record_every=every, probe_capacity=cap, raster=raster,
+                    surface=free_surface(cfg))
+
+
+FREE_SURFACES = {"reference": 0, "fixed": 1}
+
+
+def free_surface(cfg):
+    """pf_desc.surface for a solver cfg's "free_surface" entry."""
+    v = cfg.get("free_surface", "reference")
+    if v not in FREE_SURFACES:
+        raise ValueError("free_surface must be one of %s (got %r)" % (sorted(FREE_SURFACES), v))
+    return FREE_SURFACES[v]

@@ -15,6 +15,9 @@ cfg keys added to the reference's {wave, wave_args, write_mode}:
   probe         None | [i0, si, ni, j0, sj, nj] | {'stride': s} | {'count': n}
   record_every  probe frame every k steps (frames land in ``self.probes``)
   sponge        None | {'width': cells, 'strength': s}  (extension absorber)
+  free_surface  'reference' (default, bit for bit) | 'fixed': the surface u_z
+                update with its precedence corrected (base_solver.py:516), for
+                physical spacings (extension; equals 'reference' at unit spacing)
   record_slots  snapshot slots of the per-step recording (write_mode != 'off')
 write_mode adds 'npy' to the reference's 'off' / 'thread' / 'process': the
 Writer's datasets and attributes as a .npy directory at ``file + '.d'``

@@ -24,7 +24,7 @@ def declared_symbols():
 def test_library_is_built_and_loads():
     assert os.path.exists(fdtd.LIB_PATH), "run __graft_entry__.build() first"
     lib = fdtd.load()
-    assert lib.pf_abi_version() == 1
+    assert lib.pf_abi_version() == 2
 
 
 def test_every_declared_symbol_is_exported():
