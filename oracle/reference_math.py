@@ -26,9 +26,12 @@ layout (``C[nx,ny,nz,6,6]``, ``P[nx,ny,nz]``), so timing this module also
 reproduces the reference's CPU cost model.
 
 Extensions with no reference counterpart (pinned by their own tests, not by
-the reference): the ``gauss`` source, the damping ``sponge`` layer and the
-``energy`` audit.  They are defined here once and the GPU path is checked
-against these definitions.
+the reference): the ``gauss`` source, the damping ``sponge`` layer, the
+``energy`` audit and the fixed free surface (``Problem(surface_fixed=True)``:
+the surface u_z update divides the T3 difference by fdz; tests/test_surface.py
+pins it by its equality with the reference at unit spacing and its
+invariance under grid scaling).  They are defined here once and the GPU path
+is checked against these definitions.
 """
 import numpy as np
 
