@@ -494,7 +494,7 @@ template <class R, int MODE>
 static int launch_faces_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
     long long rb, eb;
     if (face_rows_grid(c->nx, c->ny, c->nz, rb, eb) && rb + eb <= 0x7fffffffLL) {
-        hipLaunchKernelGGL((k_faces_rows<R, MODE>), dim3((unsigned)(rb + eb)), dim3(256), 0, s, m, (int)eb);
+        hipLaunchKernelGGL((k_faces_rows<R, MODE>), dim3((unsigned)(rb + eb)), dim3(256), 0, s, m, (int)rb);
     } else {
         const long long n = face_total(c->nx, c->ny, c->nz);
         hipLaunchKernelGGL((k_faces<R, MODE>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m, n);

@@ -62,6 +62,30 @@ def test_load_settings_reproduces_reference_inputs(tmp_path, name):
     assert common.solver.cfg["wave"] == json.loads(str(d["cfg"]))["wave"]
 
 
+@pytest.mark.parametrize("name", ["default_json_100", "nonuniform_json_60"])
+def test_save_settings_round_trips(tmp_path, name):
+    # settings written back (common.py:158-181) load to the same mesh,
+    # material map, time step and solver cfg
+    d = golden(name)
+    first = tmp_path / "a.json"
+    first.write_text(json.dumps(_settings_from_fixture(d)))
+    common.findSolvers()
+    common.loadSettings(str(first))
+    common.solver.cfg["precision"] = "fp32c"  # a cfg change the file must carry
+    second = tmp_path / "b.json"
+    common.saveSettings(str(second))
+    saved = json.loads(second.read_text())
+    assert saved["simulation"]["solver"] == "hip"
+    assert saved["simulation"]["cfg"]["precision"] == "fp32c"
+    assert saved["material"]["primary"] == "M0" and saved["material"]["secondary"] == "M1"
+    _, g, m = common.loadSettings(str(second))
+    for a in "xyz":
+        assert np.array_equal(getattr(g, a), d[a]), a
+    assert np.array_equal(m.ids, d["ids"])
+    assert m.dt == d["dt"][()]
+    assert common.solver.cfg["precision"] == "fp32c"
+
+
 def test_test_defaults_match_reference():
     d = golden("testdefaults_10")
     s = solver_hip.Solver()
