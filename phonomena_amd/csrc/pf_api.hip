@@ -490,19 +490,6 @@ static int launch_sweep_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
     return PF_OK;
 }
 
-template <class R, int MODE>
-static int launch_faces_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
-    long long rb, eb;
-    if (face_rows_grid(c->nx, c->ny, c->nz, rb, eb) && rb + eb <= 0x7fffffffLL) {
-        hipLaunchKernelGGL((k_faces_rows<R, MODE>), dim3((unsigned)(rb + eb)), dim3(256), 0, s, m, (int)rb);
-    } else {
-        const long long n = face_total(c->nx, c->ny, c->nz);
-        hipLaunchKernelGGL((k_faces<R, MODE>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m, n);
-    }
-    HIPCHK(hipGetLastError());
-    return PF_OK;
-}
-
 template <class R>
 static int launch_probe_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
     if (c->probe[2] <= 0) return PF_OK;
@@ -519,6 +506,40 @@ static int launch_probe_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
     return PF_OK;
 }
 
+// Whether the step of view m records a probe frame: always possibly when its
+// step index is on the device (the kernel decides), else by the host index.
+template <class R>
+static bool probe_due(const pf_ctx* c, const Mesh<R>& m) {
+    if (c->probe[2] <= 0) return false;
+    return m.step_base || (m.step_off < m.nsteps && (m.step_off + 1) % c->every == 0);
+}
+
+// The faces of the level view m writes, and the step's probe frame: one
+// launch with the row-wise face kernel, which takes the probe blocks too;
+// the per-cell face kernel and a probe launch on meshes too small for it.
+template <class R, int MODE>
+static int launch_faces_probe_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
+    long long rb, eb;
+    if (face_rows_grid(c->nx, c->ny, c->nz, rb, eb)) {
+        FaceProbe pr = {};
+        pr.every = 1;
+        if (probe_due(c, m)) {
+            const int* q = c->probe;
+            pr = {c->ring, c->d_frame, (int*)(c->d_frame + 1), q[0], q[1], q[2], q[3], q[4], q[5], c->cap, c->every,
+                  (q[2] * q[5] + 255) / 256};
+        }
+        if (rb + eb + pr.blocks > 0x7fffffffLL) return fail(PF_ERR_ARG, "too many face cells");
+        hipLaunchKernelGGL((k_faces_rows<R, MODE>), dim3((unsigned)(rb + eb + pr.blocks)), dim3(256), 0, s, m, (int)rb,
+                           (int)eb, pr);
+        HIPCHK(hipGetLastError());
+        return PF_OK;
+    }
+    const long long n = face_total(c->nx, c->ny, c->nz);
+    hipLaunchKernelGGL((k_faces<R, MODE>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m, n);
+    HIPCHK(hipGetLastError());
+    return launch_probe_t<R>(c, m, s);
+}
+
 template <class R, int MODE>
 static int step_once_t(pf_ctx* c, hipStream_t s, const long long* base, long long off,
                        hipEvent_t* ev /* 2 or null */) {
@@ -529,8 +550,7 @@ static int step_once_t(pf_ctx* c, hipStream_t s, const long long* base, long lon
     if (ev) HIPCHK(hipEventRecord(ev[0], s));
     if ((rc = launch_sweep_t<R, MODE>(c, m, s))) return rc;
     if (ev) HIPCHK(hipEventRecord(ev[1], s));
-    if ((rc = launch_faces_t<R, MODE>(c, m, s))) return rc;
-    if ((rc = launch_probe_t<R>(c, m, s))) return rc;
+    if ((rc = launch_faces_probe_t<R, MODE>(c, m, s))) return rc;
     const int t = c->cur;
     c->cur = c->old;
     c->old = t;
@@ -651,8 +671,7 @@ static int step_pair_t(pf_ctx* c, hipStream_t s, const long long* base, long lon
     int rc = c->sponge ? launch_pair_s<R, MODE, true>(c, m1, m2.n0, s) : launch_pair_s<R, MODE, false>(c, m1, m2.n0, s);
     if (rc) return rc;
     if (ev) HIPCHK(hipEventRecord(ev[1], s));
-    if ((rc = launch_faces_t<R, MODE>(c, m1, s))) return rc;
-    if ((rc = launch_probe_t<R>(c, m1, s))) return rc;
+    if ((rc = launch_faces_probe_t<R, MODE>(c, m1, s))) return rc;
     const BandMap bm = band_map(c);
     const long long nb = band_blocks(bm);
     if (nb > 0x7fffffffLL) return fail(PF_ERR_ARG, "too many tiles");
@@ -661,8 +680,7 @@ static int step_pair_t(pf_ctx* c, hipStream_t s, const long long* base, long lon
     else
         hipLaunchKernelGGL((k_band<R, MODE, false>), dim3((unsigned)nb), dim3(64, 4), 0, s, m2, bm.j, bm.i);
     HIPCHK(hipGetLastError());
-    if ((rc = launch_faces_t<R, MODE>(c, m2, s))) return rc;
-    if ((rc = launch_probe_t<R>(c, m2, s))) return rc;
+    if ((rc = launch_faces_probe_t<R, MODE>(c, m2, s))) return rc;
     c->old = a1;
     c->cur = a2;
     return PF_OK;

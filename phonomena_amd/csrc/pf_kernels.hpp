@@ -431,13 +431,15 @@ PF_DEV int face_rows(int nx, int ny) {
 template <int Q, int MODE, class R>
 PF_DEV void face_row_run(const Mesh<R>& m, int r, int run, int lane, R w) {
     const int Lx = Ext<Q>::Lx(m.nx), Ly = Ext<Q>::Ly(m.ny), Lz = Ext<Q>::Lz(m.nz);
-    int i, j, ii, ji, f;  // the face row, the interior row its map reads, the face index
+    int i, j, ii, ji;  // the face row, the interior row its map reads
+    R mh, ml;          // its Mur coefficient (constant indices: a run-time index
+                       // into the kernel argument would copy it to scratch)
     if (r < Ly - 1) {
-        i = Lx; j = 1 + r; ii = Lx - 1; ji = j; f = 0;
+        i = Lx; j = 1 + r; ii = Lx - 1; ji = j; mh = m.mur[0 + Q]; ml = m.mur_lo[0 + Q];
     } else if (r < Ly - 1 + Lx) {
-        i = r - (Ly - 1); j = 0; ii = i; ji = 1; f = 3;
+        i = r - (Ly - 1); j = 0; ii = i; ji = 1; mh = m.mur[3 + Q]; ml = m.mur_lo[3 + Q];
     } else {
-        i = r - (Ly - 1) - Lx; j = Ly; ii = i; ji = Ly - 1; f = 6;
+        i = r - (Ly - 1) - Lx; j = Ly; ii = i; ji = Ly - 1; mh = m.mur[6 + Q]; ml = m.mur_lo[6 + Q];
     }
     const R* c = Q == 0 ? m.c0 : (Q == 1 ? m.c1 : m.c2);
     const R* n = Q == 0 ? m.n0 : (Q == 1 ? m.n1 : m.n2);
@@ -468,7 +470,7 @@ PF_DEV void face_row_run(const Mesh<R>& m, int r, int run, int lane, R w) {
             if (ii == 0) { a[u].h = w; a[u].l = R(0); }
             if (i == 0) { d[u].h = w; d[u].l = R(0); }
         }
-        put<Q, MODE>(m, i, j, k, mur_map<MODE>(a[u], m.mur[f + Q], m.mur_lo[f + Q], b[u], d[u]));
+        put<Q, MODE>(m, i, j, k, mur_map<MODE>(a[u], mh, ml, b[u], d[u]));
     }
 }
 
@@ -478,11 +480,60 @@ PF_DEV long long face_edge_base(int nx, int ny, int nz) {
     return (Ly - 1) * Lz + 2 * Lx * Lz;
 }
 
+// A probe frame recorded by the face launch (probe blocks after the row
+// and edge blocks; blocks = 0: none).  The frame index advances as in
+// k_probe.
+struct FaceProbe {
+    double* ring;
+    long long* frame;
+    int* done;
+    int i0, si, ni, j0, sj, nj, cap, every, blocks;
+};
+
+// Probe point p of the u_z surface (k = 0) of the level m writes, as a later
+// read would see it.  Points on a u_z face (i = Lx, j = 0 or Ly) are not
+// read: the face blocks of the same launch store them, so the point takes
+// the same chain value (final_value) with its lo part rounded as stored.
 template <class R, int MODE>
-__global__ __launch_bounds__(256) void k_faces_rows(Mesh<R> m, int row_blocks) {
+PF_DEV double probe_point(const Mesh<R>& m, const FaceProbe& pr, int p, R w) {
+    const int i = pr.i0 + pr.si * (p / pr.nj), j = pr.j0 + pr.sj * (p % pr.nj);
+    if (i == Ext<2>::Lx(m.nx) || j == 0 || j == Ext<2>::Ly(m.ny)) {
+        const V<R> v = final_value<2, MODE>(m, i, j, 0, w);
+        double d = (double)v.h;
+        if constexpr (MODE == MODE_FP32C) d += (double)lo_decode(v.h, lo_encode(v.h, v.l));
+        return d;
+    }
+    const long long id = at(m, i, j, 0);
+    const R h = m.n2[id];
+    double d = (double)h;
+    if (m.nl2) d += (double)lo_decode(h, m.nl2[id]);
+    return d;
+}
+
+template <class R, int MODE>
+__global__ __launch_bounds__(256) void k_faces_rows(Mesh<R> m, int row_blocks, int edge_blocks, FaceProbe pr) {
+    __shared__ int s_last;
     const long long tt = step_index(m);
     if (tt >= m.nsteps) return;
     const R w = (R)m.wave[tt];
+    if ((int)blockIdx.x >= row_blocks + edge_blocks) {
+        // probe blocks: the frame of a recording step (the same for every block)
+        if ((tt + 1) % pr.every != 0) return;
+        const long long f = *pr.frame;
+        const int p = ((int)blockIdx.x - row_blocks - edge_blocks) * 256 + (int)threadIdx.x;
+        if (p < pr.ni * pr.nj) pr.ring[(f % pr.cap) * (long long)pr.ni * pr.nj + p] = probe_point<R, MODE>(m, pr, p, w);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __threadfence();
+            s_last = atomicAdd(pr.done, 1) == pr.blocks - 1;
+        }
+        __syncthreads();
+        if (s_last && threadIdx.x == 0) {
+            *pr.frame = f + 1;
+            *pr.done = 0;
+        }
+        return;
+    }
     if ((int)blockIdx.x < row_blocks) {
         const int kr = face_runs(m.nz);
         const int lane = threadIdx.x % 64;
