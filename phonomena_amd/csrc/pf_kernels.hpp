@@ -510,6 +510,35 @@ PF_DEV double probe_point(const Mesh<R>& m, const FaceProbe& pr, int p, R w) {
     return d;
 }
 
+// Row block rb of k_faces_rows' grid for view m (four runs, a wave each).
+template <class R, int MODE>
+PF_DEV void face_rows_block(const Mesh<R>& m, int rb, R w) {
+    const int kr = face_runs(m.nz);
+    const int lane = threadIdx.x % 64;
+    int t = rb * 4 + (int)threadIdx.x / 64;  // this wave's run
+    const int w0 = face_rows<0>(m.nx, m.ny) * kr;
+    if (t < w0) { face_row_run<0, MODE>(m, t / kr, t % kr, lane, w); return; }
+    t -= w0;
+    const int w1 = face_rows<1>(m.nx, m.ny) * kr;
+    if (t < w1) { face_row_run<1, MODE>(m, t / kr, t % kr, lane, w); return; }
+    t -= w1;
+    if (t < face_rows<2>(m.nx, m.ny) * kr) face_row_run<2, MODE>(m, t / kr, t % kr, lane, w);
+}
+
+// Edge block eb (256 box-edge cells, one a thread) for view m.
+template <class R, int MODE>
+PF_DEV void face_edges_block(const Mesh<R>& m, int eb, R w) {
+    long long e = (long long)eb * 256 + threadIdx.x;
+    const long long e0 = face_count<0>(m.nx, m.ny, m.nz) - face_edge_base<0>(m.nx, m.ny, m.nz);
+    if (e < e0) { face_write<0, MODE>(m, face_edge_base<0>(m.nx, m.ny, m.nz) + e, w); return; }
+    e -= e0;
+    const long long e1 = face_count<1>(m.nx, m.ny, m.nz) - face_edge_base<1>(m.nx, m.ny, m.nz);
+    if (e < e1) { face_write<1, MODE>(m, face_edge_base<1>(m.nx, m.ny, m.nz) + e, w); return; }
+    e -= e1;
+    const long long e2 = face_count<2>(m.nx, m.ny, m.nz) - face_edge_base<2>(m.nx, m.ny, m.nz);
+    if (e < e2) face_write<2, MODE>(m, face_edge_base<2>(m.nx, m.ny, m.nz) + e, w);
+}
+
 template <class R, int MODE>
 __global__ __launch_bounds__(256) void k_faces_rows(Mesh<R> m, int row_blocks, int edge_blocks, FaceProbe pr) {
     __shared__ int s_last;
@@ -534,28 +563,8 @@ __global__ __launch_bounds__(256) void k_faces_rows(Mesh<R> m, int row_blocks, i
         }
         return;
     }
-    if ((int)blockIdx.x < row_blocks) {
-        const int kr = face_runs(m.nz);
-        const int lane = threadIdx.x % 64;
-        int t = (int)blockIdx.x * 4 + (int)threadIdx.x / 64;  // this wave's run
-        const int w0 = face_rows<0>(m.nx, m.ny) * kr;
-        if (t < w0) { face_row_run<0, MODE>(m, t / kr, t % kr, lane, w); return; }
-        t -= w0;
-        const int w1 = face_rows<1>(m.nx, m.ny) * kr;
-        if (t < w1) { face_row_run<1, MODE>(m, t / kr, t % kr, lane, w); return; }
-        t -= w1;
-        if (t < face_rows<2>(m.nx, m.ny) * kr) face_row_run<2, MODE>(m, t / kr, t % kr, lane, w);
-        return;
-    }
-    long long e = (long long)((int)blockIdx.x - row_blocks) * 256 + threadIdx.x;
-    const long long e0 = face_count<0>(m.nx, m.ny, m.nz) - face_edge_base<0>(m.nx, m.ny, m.nz);
-    if (e < e0) { face_write<0, MODE>(m, face_edge_base<0>(m.nx, m.ny, m.nz) + e, w); return; }
-    e -= e0;
-    const long long e1 = face_count<1>(m.nx, m.ny, m.nz) - face_edge_base<1>(m.nx, m.ny, m.nz);
-    if (e < e1) { face_write<1, MODE>(m, face_edge_base<1>(m.nx, m.ny, m.nz) + e, w); return; }
-    e -= e1;
-    const long long e2 = face_count<2>(m.nx, m.ny, m.nz) - face_edge_base<2>(m.nx, m.ny, m.nz);
-    if (e < e2) face_write<2, MODE>(m, face_edge_base<2>(m.nx, m.ny, m.nz) + e, w);
+    if ((int)blockIdx.x < row_blocks) face_rows_block<R, MODE>(m, (int)blockIdx.x, w);
+    else face_edges_block<R, MODE>(m, (int)blockIdx.x - row_blocks, w);
 }
 
 // Blocks of k_faces_rows for an nx x ny x nz mesh: row_blocks (four runs
