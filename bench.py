@@ -338,6 +338,39 @@ def bench_precision(steps, workload="plain"):
     return out
 
 
+def bench_physical(steps=1000):
+    """The plain CPU case at its physical spacing: 64x64x32 cells at
+    dx = 1e-4 m, homogeneous (rho 2700, lambda 5.1e10, mu 2.6e10), Courant
+    0.5, 1 MHz sin source on the x = 0 wall, 8-cell absorbing layer, surface
+    u_z every 10 steps, with the fixed free surface (the reference's
+    precedence diverges at such a spacing).  FP32 and FP32C against FP64 of
+    the same scheme, per field and probe frames; and FP64 against the
+    reference's scheme run at unit spacing with the same source sequence
+    (equal up to rounding: the fixed scheme does not depend on the spacing)."""
+    from phonomena_amd import fdtd, sweep
+    from phonomena_amd.simulation import lattice
+    kw = dict(inclusion=lattice.MATRIX, probe=[0, 2, 32, 0, 2, 32], every=10, sponge={"width": 8},
+              kind="sin", f0=1e6, dx_phys=1e-4)
+    phys, _, _ = sweep.crystal_spec((64, 64, 32), 16, 5, 0, steps, d=1e-4, free_surface="fixed", **kw)
+    unit, _, _ = sweep.crystal_spec((64, 64, 32), 16, 5, 0, steps, d=1.0, **kw)
+    unit.wave = phys.wave
+
+    def run(spec, prec):
+        c = fdtd.Context(spec, precision=prec)
+        c.step(steps)
+        out = {q: c.get(q) for q in ("ux", "uy", "uz")}
+        out["probes"] = c.probes()
+        c.close()
+        return out
+
+    ref = run(phys, "fp64")
+    rel = lambda got: {q: sweep.rel_l2(got[q], ref[q]) for q in ref}
+    return {"workload": "64x64x32 cells at dx = 1e-4 m, homogeneous, Courant 0.5, 1 MHz sin, 8-cell layer, "
+                        "%d steps, fixed free surface" % steps,
+            "fp64_vs_unit_spacing_reference": rel(run(unit, "fp64")),
+            "fp32_vs_fp64": rel(run(phys, "fp32")), "fp32c_vs_fp64": rel(run(phys, "fp32c"))}
+
+
 def bench_served(n_jobs, capacity_jobs_per_s, precision="fp32c", load=0.7, seed=5, chunk=200):
     """Served design-sweep stream: Poisson arrivals at `load` x the batched
     capacity (jobs/s), grids U[32^3 .. 256x256x128] cells, U{500..8000}
@@ -378,6 +411,8 @@ def main():
                     help="steps of the FP32C/FP32 vs FP64 precision case (0 = skip)")
     ap.add_argument("--serve-jobs", type=int, default=48, help="jobs in the served stream")
     ap.add_argument("--no-setup", action="store_true", help="skip the mesh set-up (material map) leg")
+    ap.add_argument("--no-physical", action="store_true",
+                    help="skip the physical-spacing leg (plain CPU case at dx = 1e-4 m, fixed free surface)")
     ap.add_argument("--no-precisions", action="store_true",
                     help="skip the FP32C / FP64 legs of the plain workload")
     ap.add_argument("--no-plain", action="store_true",
@@ -405,6 +440,8 @@ def main():
         rec["precision_case"] = bench_precision(args.precision_steps)
     if not args.no_setup:
         rec["setup"] = bench_setup(args.workload)
+    if not args.no_physical:
+        rec["physical_spacing"] = bench_physical()
     rec["cpu_baseline"] = None if args.no_cpu else cpu_baseline()
     if not args.no_cpu:
         rec["cpu_baseline_omp"] = cpu_baseline_omp()

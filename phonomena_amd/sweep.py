@@ -22,18 +22,24 @@ INCLUSIONS = {"void": lattice.VOID, "tungsten": lattice.TUNGSTEN}
 
 
 def crystal_spec(cells, a, r, seed, steps, inclusion="tungsten", probe=None, every=10, sponge=None,
-                 kind="gauss", capacity=None, device_ids=False):
+                 kind="gauss", capacity=None, device_ids=False, d=1.0, f0=F0, dx_phys=DX_PHYS,
+                 free_surface="reference"):
     """(MeshSpec, Grid, Material) of one crystal job.  device_ids: the
     material map is rasterized on the device (the Material's map is left
-    empty)."""
+    empty).  d: the cell size in the grid's coordinates (1 = unit cells; a
+    physical size such as dx_phys needs free_surface "fixed", see
+    problem.free_surface); the source frequency is f0 * dx_phys / d per unit
+    time, the same per step at every d.  inclusion: a name of INCLUSIONS or
+    a (name, rho, lambda, mu) tuple (the matrix's own gives a homogeneous
+    box)."""
     inc = INCLUSIONS[inclusion] if isinstance(inclusion, str) else inclusion
     g, m = lattice.crystal(tuple(int(c) for c in cells), float(a), float(r), int(seed), inclusion=inc,
-                           host_ids=not device_ids)
+                           host_ids=not device_ids, d=float(d))
     b = BaseSolver(None)
     b.m = m
-    b.cfg["wave"], b.cfg["wave_args"] = kind, {"f": F0 * DX_PHYS}
+    b.cfg["wave"], b.cfg["wave_args"] = kind, {"f": f0 * dx_phys / float(d)}
     wave = b.source_table(int(steps))
-    cfg = {"probe": probe, "record_every": every, "sponge": sponge}
+    cfg = {"probe": probe, "record_every": every, "sponge": sponge, "free_surface": free_surface}
     if capacity:
         cfg["probe_capacity"] = int(capacity)
     spec = problem.mesh_spec(g, m, wave, cfg, device_ids=device_ids)

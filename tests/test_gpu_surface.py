@@ -90,3 +90,28 @@ def test_c1_case_at_physical_spacing():
             r = rel_l2(got[f], phys[f])
             print(precision, "vs fp64, fixed surface at 1e-4 m", f, r)
             assert r < tol, (precision, f, r)
+
+
+def test_batch_members_keep_their_surface():
+    # a fixed-surface member at a physical spacing beside a reference-surface
+    # member at unit spacing: each equals its alone run bit for bit
+    steps = 32
+    specs = [_box_spec(1e-4, steps), _box_spec(1.0, steps)]
+    specs[1].surface = 0
+    for precision in ("fp64", "fp32c"):
+        ctxs = [fdtd.Context(s, precision=precision, tune=False) for s in specs]
+        b = fdtd.Batch(precision)
+        for c in ctxs:
+            b.add(c)
+        b.step(steps)
+        b.sync()
+        batched = [{f: c.get(f) for f in FIELDS} for c in ctxs]
+        b.close()
+        for c in ctxs:
+            c.close()
+        for s, got in zip(specs, batched):
+            ctx = fdtd.Context(s, precision=precision)
+            ctx.step(steps)
+            for f in FIELDS:
+                assert np.array_equal(ctx.get(f), got[f]), (precision, s.surface, f)
+            ctx.close()
