@@ -247,3 +247,55 @@ def test_energy_conserved_in_closed_box_without_source():
     es = np.array(es)
     assert es.min() > 0
     assert (es.max() - es.min()) / es.mean() < 1e-8
+
+
+def _courant_one_pulse(steps, mur_x=None, n=(50, 100, 100)):
+    """A P pulse, uniform in y and z, moving +x at Courant 1 (dt = h / v_p)
+    in a homogeneous box; ux on the box's centre line after every step (C
+    oracle).  mur_x replaces the x = max face's ux coefficient."""
+    x, y, z = (np.arange(k, dtype=np.float64) for k in n)
+    rho, lam, mu = 2700.0, 5.1e10, 2.6e10
+    c = np.zeros((6, 6))
+    c[:3, :3] = lam
+    for a in range(3):
+        c[a, a] = lam + 2 * mu
+        c[3 + a, 3 + a] = mu
+    table = brute.material_table([c], [rho])
+    ids = np.zeros(n, dtype=np.uint8)
+    dt = 1.0 / np.sqrt((lam + 2 * mu) / rho)
+    pb = rm.Problem(x, y, z, *rm.dense_material(ids, [c], [rho]), dt, np.zeros(steps))
+    mc = pb.mur_c
+    mur12 = np.array(list(mc["xmax"]) + list(mc["y0"]) + list(mc["ymax"]) + list(mc["zmax"]))
+    if mur_x is not None:
+        mur12[0] = mur_x
+    xs = np.arange(n[0] - 1) + 0.5
+    f = lambda X: np.exp(-((X - 15) / 2.0) ** 2)
+    shp = {"x": (n[0] - 1, n[1], n[2]), "y": (n[0], n[1] - 1, n[2]), "z": (n[0], n[1], n[2] - 1)}
+    st = {"u" + q + suf: np.zeros(shp[q]) for q in "xyz" for suf in ("", "_new", "_old")}
+    st["ux"][...] = f(xs)[:, None, None]
+    st["ux_old"][...] = f(xs + 1)[:, None, None]  # one step earlier: moving +x
+    st["ux_new"][...] = st["ux"]
+    line = []
+    for tt in range(steps):
+        st, _ = brute.run(x, y, z, ids, table, dt, mur12, pb.wave, 1, state=st, t0=tt, omp=True)
+        line.append(st["ux"][:, n[1] // 2, n[2] // 2].copy())
+    return np.array(line), f, xs, (dt * np.sqrt((lam + 2 * mu) / rho), mc, np.sqrt(mu / rho) * dt)
+
+
+def test_mur_absorbs_a_normal_plane_wave_exactly_at_courant_one():
+    # At Courant 1 the 1-D leapfrog shifts a wave one cell per step exactly,
+    # and the first-order Mur coefficient (v dt - h) / (v dt + h) is 0, so
+    # the face passes the wave out with no reflection (base_solver.py:
+    # 525-544).  On the box's centre line the 3-D scheme is that 1-D scheme
+    # for as many steps as the line is cells away from the y and z faces and
+    # the surface (domain of dependence).
+    steps = 44
+    line, f, xs, (nu, mc, vt_dt) = _courant_one_pulse(steps)
+    assert nu == pytest.approx(1.0, abs=1e-15) and abs(mc["xmax"][0]) < 1e-15
+    for s in (10, 20, 30):
+        assert np.abs(line[s - 1] - f(xs - s)).max() < 1e-12, s  # exact shift
+    assert np.abs(line[-1]).max() < 1e-10  # the pulse has left: nothing came back
+    # planted mistake: the shear speed in the x face's ux coefficient
+    wrong = (vt_dt - 1.0) / (vt_dt + 1.0)
+    bad, _, _, _ = _courant_one_pulse(steps, mur_x=wrong)
+    assert np.abs(bad[-1]).max() > 0.1
