@@ -216,12 +216,15 @@ int pf_dft(const double *in_re, const double *in_im, int64_t batch, int64_t n, i
 /*
  * Timing: run nsteps steps on the context's stream with HIP events around
  * the sweep launch of every fourth step (with PF_KERNEL_TILED2, of every
- * fourth pair of steps, whose sweep is the two-step pass), the first
- * included, and one after the last step; returns the mean duration per
- * step of the sweeps (sweep_ms) and of everything else (face_ms: faces,
- * probes, launch gaps; with PF_KERNEL_TILED2 also the band, the level n+2
- * cells next to the x and y faces that the two-step pass leaves).  Each
- * event delays the launch after it by a few microseconds, hence the stride.
+ * fourth pair of steps, whose sweep is the two-step pass; a wider stride
+ * where that would bracket more than eight), the first included, and one
+ * after the last step; returns the mean duration per step of the sweeps
+ * (sweep_ms) and of everything else (face_ms: faces, probes, launch gaps;
+ * with PF_KERNEL_TILED2 also the band, the level n+2 cells next to the x
+ * and y faces that the two-step pass leaves).  Each event delays the
+ * launch after it by a few microseconds, hence the stride.  The steps and
+ * their results are those of pf_step.  The first call on a context creates
+ * its timing events (33), which later calls reuse and pf_destroy frees.
  */
 int pf_profile(pf_ctx *ctx, int64_t nsteps, double *sweep_ms, double *face_ms);
 

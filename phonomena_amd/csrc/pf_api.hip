@@ -425,6 +425,8 @@ void pf::ctx_free(pf_ctx* c) {
         if (c->snap_copied[i]) (void)hipEventDestroy(c->snap_copied[i]);
     }
     if (c->snap_stream) (void)hipStreamDestroy(c->snap_stream);
+    for (auto& e : c->prof_ev)
+        if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -1051,30 +1053,45 @@ extern "C" int pf_profile(pf_ctx* c, int64_t nsteps, double* sweep_ms, double* f
     if (nsteps < 1) return fail(PF_ERR_ARG, "nsteps < 1");
     if (c->step + nsteps > c->wave_len) return fail(PF_ERR_RANGE, "beyond the source table");
     // Launch groups: a pair's two-step pass with its band, faces and probes,
-    // or a single step.  Events bracket the sweep of every fourth group of
-    // each kind (the first included), and one event follows the last group:
-    // a recorded event is a marker between two launches that holds the next
-    // launch back by a few microseconds, so bracketing every group would
-    // slow the steps it times.  Sweep time = the mean bracketed sweep of
-    // each kind times the groups of that kind; the rest (faces, probes, the
-    // band of a pair, launch gaps) = the span from the first event to the
-    // last less the sweeps.
-    const int64_t stride = 4;
+    // or a single step.  Events bracket the sweep of every stride-th group
+    // of each kind (the first included; stride 4, or more so that at most
+    // PF_PROFILE_BRACKETS groups of a kind are bracketed), and one event
+    // follows the last group: a recorded event is a marker between two
+    // launches that holds the next launch back by a few microseconds, so
+    // bracketing every group would slow the steps it times.  Sweep time =
+    // the mean bracketed sweep of each kind times the groups of that kind;
+    // the rest (faces, probes, the band of a pair, launch gaps) = the span
+    // from the first event to the last less the sweeps.  The events are the
+    // context's own, created by the first call.
     const int64_t pairs = c->kernel == PF_KERNEL_TILED2 ? nsteps / 2 : 0;
     const int64_t singles = nsteps - 2 * pairs;
-    std::vector<hipEvent_t> ev((size_t)((pairs + stride - 1) / stride + (singles + stride - 1) / stride) * 2 + 1);
-    for (auto& e : ev) HIPCHK(hipEventCreateWithFlags(&e, PF_TIMING_EVENT));
+    auto stride_of = [](int64_t groups) {
+        const int64_t s = (groups + PF_PROFILE_BRACKETS - 1) / PF_PROFILE_BRACKETS;
+        return s > 4 ? s : (int64_t)4;
+    };
+    const int64_t pstride = stride_of(pairs), sstride = stride_of(singles);
+    hipEvent_t* ev = c->prof_ev;
+    if (!ev[PF_PROFILE_EVENTS - 1]) {
+        for (auto& e : c->prof_ev) {
+            if (e) continue;
+            const hipError_t err = hipEventCreateWithFlags(&e, PF_TIMING_EVENT);
+            if (err != hipSuccess) {
+                e = nullptr;
+                return fail(PF_ERR_HIP, "hipEventCreateWithFlags: %s", hipGetErrorString(err));
+            }
+        }
+    }
     int rc = PF_OK;
     size_t used = 0;
     for (int64_t g = 0; g < pairs && !rc; g++) {
-        hipEvent_t* e = g % stride == 0 ? &ev[used] : nullptr;
+        hipEvent_t* e = g % pstride == 0 ? &ev[used] : nullptr;
         if (e) used += 2;
         rc = step_pair(c, c->stream, nullptr, c->step, e);
         if (!rc) c->step += 2;
     }
     const size_t pair_ev = used;
     for (int64_t g = 0; g < singles && !rc; g++) {
-        hipEvent_t* e = g % stride == 0 ? &ev[used] : nullptr;
+        hipEvent_t* e = g % sstride == 0 ? &ev[used] : nullptr;
         if (e) used += 2;
         rc = step_once(c, c->stream, nullptr, c->step, e);
         if (!rc) c->step++;
@@ -1095,6 +1112,5 @@ extern "C" int pf_profile(pf_ctx* c, int64_t nsteps, double* sweep_ms, double* f
         *sweep_ms = sw / (double)nsteps;
         *face_ms = ((double)all - sw) / (double)nsteps;
     }
-    for (auto& e : ev) (void)hipEventDestroy(e);
     return rc;
 }

@@ -16,6 +16,10 @@
 #define PF_TILED_MAX_MATERIALS 16
 // Snapshot slots a context can hold.
 #define PF_SNAP_MAX_SLOTS 16
+// Launch groups of each kind (pairs, single steps) that one pf_profile call
+// brackets with events at most.
+#define PF_PROFILE_BRACKETS 8
+#define PF_PROFILE_EVENTS (4 * PF_PROFILE_BRACKETS + 1)
 
 // One mesh on one device.  Time levels of the displacement components are
 // kept in level buffers: `cur` indexes the current level (n) and `old` the
@@ -79,6 +83,9 @@ struct pf_ctx {
     hipEvent_t snap_copied[PF_SNAP_MAX_SLOTS] = {};
     int snap_state[PF_SNAP_MAX_SLOTS] = {};      // 0 free, 1 in flight, 2 held by the caller
     long long snap_step[PF_SNAP_MAX_SLOTS] = {};
+    // pf_profile's timing events, created by its first call and kept, so
+    // that later calls time the steps and not the event set-up
+    hipEvent_t prof_ev[PF_PROFILE_EVENTS] = {};
 };
 
 namespace pf {

@@ -6,9 +6,10 @@ import numpy as np
 
 from oracle import reference_math as rm
 from oracle import brute
-from phonomena_amd import problem
+from phonomena_amd import fdtd, problem
 from phonomena_amd.fdtd import MeshSpec
 
+FIELDS = ["ux", "uy", "uz", "ux_old", "uy_old", "uz_old"]
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 CASES = ["testdefaults_10", "homog16_200", "default_json_100", "nonuniform_json_60", "fine_json_20"]
 
@@ -185,3 +186,28 @@ def reference_style_material(grid, d):
     m.setPrimary("M0")
     m.setSecondary("M1")
     return m
+
+
+def random_start(nodes, seed):
+    """Random level n (and n-1 = half of it) of every component; the x = 0
+    wall of uy, uz stays 0 as in the reference."""
+    rng = np.random.default_rng(seed)
+    start = {c: rng.standard_normal(s) for c, s in (("ux", (nodes[0] - 1,) + tuple(nodes[1:])),
+                                                   ("uy", (nodes[0], nodes[1] - 1, nodes[2])),
+                                                   ("uz", (nodes[0], nodes[1], nodes[2] - 1)))}
+    for c in ("uy", "uz"):
+        start[c][0] = 0.0
+    return start
+
+
+def run_from(spec, start, precision, kernel, steps, **tiles):
+    ctx = fdtd.Context(spec, precision=precision, kernel=kernel, **tiles)
+    for c in ("ux", "uy", "uz"):
+        ctx.set(c, start[c])
+        ctx.set(c + "_old", 0.5 * start[c])
+    ctx.step(steps)
+    ctx.sync()
+    out = {f: ctx.get(f) for f in FIELDS}
+    out["probes"] = ctx.probes()
+    ctx.close()
+    return out

@@ -18,7 +18,7 @@ import copy
 import numpy as np
 import pytest
 
-from helpers import spec_from, uniform_crystal
+from helpers import FIELDS, random_start, run_from, spec_from, uniform_crystal
 from phonomena_amd import fdtd, sweep
 
 pytestmark = pytest.mark.gpu
@@ -43,6 +43,20 @@ def test_c3_doubled_source_doubles_fields(precision, kernel):
         scale = float(np.abs(out[1][f]).max())
         assert scale > 0, f
         assert float(np.abs(2.0 * out[0][f] - out[1][f]).max()) <= 1e-7 * scale, f
+
+
+def test_c3_pair_equals_one_step_from_random_start():
+    # the two-step pass at this size (its tiles and x chunks) against the
+    # one-step sweep, bit for bit, from a random start so that every cell
+    # carries values from the first step
+    steps = 6
+    spec, _, _ = sweep.crystal_spec((512, 512, 256), 24, 8, 3, steps + 1, sponge={"width": 16},
+                                    probe={"count": 64}, every=2, device_ids=True)
+    start = random_start(spec.nodes, 5)
+    a = run_from(spec, start, "fp32", "tiled", steps)
+    b = run_from(spec, start, "fp32", "tiled2", steps)
+    for f in ["probes"] + FIELDS:
+        assert np.array_equal(a[f], b[f]), f
 
 
 def test_c2_homogeneous_box_is_mirror_symmetric_in_y():

@@ -6,11 +6,10 @@ chunkings, with and without the sponge."""
 import numpy as np
 import pytest
 
-from helpers import CASES, golden, spec_from, spec_from_golden, uniform_crystal
+from helpers import CASES, FIELDS, golden, random_start, run_from, spec_from, spec_from_golden, uniform_crystal
 from phonomena_amd import fdtd, problem, sweep
 
 pytestmark = pytest.mark.gpu
-FIELDS = ["ux", "uy", "uz", "ux_old", "uy_old", "uz_old"]
 
 
 def _run(spec, precision, kernel, chunks, **tiles):
@@ -62,6 +61,47 @@ def test_pair_single_steps_between_pairs(precision):
     spec = spec_from(x, y, z, ids, rows, dt, wave, probe=(1, 2, 15, 0, 3, 11), every=3)
     a = _run(spec, precision, "tiled", [60])
     b = _run(spec, precision, "tiled2", [1, 6, 3, 1, 1, 17, 2, 29])
+    _same(a, b)
+
+
+# Meshes whose last tile along y or z holds a few cells (65 nodes: 32-wide
+# tiles own 0..27, 28..55 and the last one 56..64) or most of its lanes, in
+# both tile shapes; from a random start, so every edge cell carries values
+# from the first step: the one-step sweep's bits.
+@pytest.mark.parametrize("cells", [(40, 64, 64), (40, 45, 64), (40, 64, 45), (40, 45, 45), (40, 64, 70),
+                                   (40, 70, 64)])
+@pytest.mark.parametrize("tile_j,tile_z", [(32, 32), (16, 64)])
+@pytest.mark.parametrize("precision", ["fp32", "fp32c"])
+def test_pair_edge_tiles_from_random_start(cells, tile_j, tile_z, precision):
+    x, y, z, ids, rows, dt, wave = uniform_crystal(
+        cells, 1.0, incl=(19300.0, 2.0e11, 1.6e11, 14, 5), steps=12, seed=4)
+    nodes = (x.size, y.size, z.size)
+    sponge = problem.sponge_tables(nodes, {"width": 6, "strength": 0.15})
+    spec = spec_from(x, y, z, ids, rows, dt, wave, probe=(0, 3, 13, 1, 4, 11), every=3, sponge=sponge)
+    start = random_start(nodes, 11)
+    a = run_from(spec, start, precision, "tiled", 11)
+    b = run_from(spec, start, precision, "tiled2", 11, tile_j=tile_j, tile_z=tile_z)
+    _same(a, b)
+
+
+@pytest.mark.parametrize("kernel", ["tiled", "tiled2"])
+def test_profile_steps_like_step(kernel):
+    # pf_profile issues pf_step's launches with timing events between some
+    # of them: the same fields and probe frames.  Its first call creates
+    # the context's events and later calls reuse them; 80 steps are 40
+    # pairs, bracketed every fifth to stay within the eight brackets.
+    x, y, z, ids, rows, dt, wave = uniform_crystal(
+        (40, 33, 50), 1.0, incl=(19300.0, 2.0e11, 1.6e11, 10, 3), steps=157, seed=2)
+    spec = spec_from(x, y, z, ids, rows, dt, wave, probe=(1, 2, 15, 0, 3, 11), every=3)
+    a = _run(spec, "fp32", kernel, [157])
+    ctx = fdtd.Context(spec, precision="fp32", kernel=kernel)
+    for n in (5, 1, 80, 71):
+        sweep_ms, face_ms = ctx.profile(n)
+        assert sweep_ms > 0 and sweep_ms + face_ms > 0
+    ctx.sync()
+    b = {f: ctx.get(f) for f in FIELDS}
+    b["probes"] = ctx.probes()
+    ctx.close()
     _same(a, b)
 
 
