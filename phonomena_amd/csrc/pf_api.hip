@@ -316,15 +316,23 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
 
     // fields: two levels (four for the two-step sweep) x three components
-    // (+ 16-bit lo parts in compensated mode), each level one allocation
+    // (+ 16-bit lo parts in compensated mode), each level one allocation.
+    // Level l starts l * PF_LEVEL_STAGGER bytes into its allocation.  The
+    // allocations' bases are aligned alike, so the same node of every level
+    // would sit at the same low address bits, and a sweep reads and writes
+    // the levels at neighbouring nodes together; an odd multiple of 256 B
+    // puts them on different memory channels (measured on the 513x513x257-
+    // node crystal: the two-step pass 1.3-1.5 % faster with 256, 2304, 4352
+    // or 8448 B, no gain with 1024, 4096 or 4608; the FP32C one-step sweep
+    // 0.4 %, FP64 unchanged).
     const size_t bytes = (size_t)c->cells * c->esz;
     const size_t lbytes = (size_t)c->cells * sizeof(lo_t);
     const size_t lvl_bytes = level_bytes(c->cells, c->precision);
     c->nlevels = pair ? 4 : 2;
     for (int l = 0; l < c->nlevels; l++) {
-        HIPCHK(hipMalloc(&c->lvlbase[l], lvl_bytes));
-        HIPCHK(hipMemsetAsync(c->lvlbase[l], 0, lvl_bytes, c->stream));
-        char* b = (char*)c->lvlbase[l];
+        HIPCHK(hipMalloc(&c->lvlbase[l], lvl_bytes + (size_t)l * PF_LEVEL_STAGGER));
+        char* b = (char*)c->lvlbase[l] + (size_t)l * PF_LEVEL_STAGGER;
+        HIPCHK(hipMemsetAsync(b, 0, lvl_bytes, c->stream));
         for (int q = 0; q < 3; q++) {
             c->lvl[l][q] = b + q * bytes;
             if (c->precision == PF_FP32C) c->lvlo[l][q] = b + 3 * bytes + q * lbytes;
@@ -620,7 +628,7 @@ static int tune_shape_t(pf_ctx* c) {
     (void)hipEventDestroy(b);
     if (rc) return rc;
     const size_t lb = level_bytes(c->cells, c->precision);
-    for (int l = 0; l < c->nlevels; l++) HIPCHK(hipMemsetAsync(c->lvlbase[l], 0, lb, c->stream));
+    for (int l = 0; l < c->nlevels; l++) HIPCHK(hipMemsetAsync(c->lvl[l][0], 0, lb, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return PF_OK;
 }

@@ -16,6 +16,9 @@
 #define PF_TILED_MAX_MATERIALS 16
 // Snapshot slots a context can hold.
 #define PF_SNAP_MAX_SLOTS 16
+// Offset of time level l in its allocation: l * PF_LEVEL_STAGGER bytes (an
+// odd multiple of 256).
+#define PF_LEVEL_STAGGER 4352
 // Launch groups of each kind (pairs, single steps) that one pf_profile call
 // brackets with events at most.
 #define PF_PROFILE_BRACKETS 8
@@ -27,7 +30,8 @@
 // swaps the two.  Two-step sweeps use two more level buffers (allocated on
 // first use).  Each level is one allocation holding its arrays back to back
 // (ux, uy, uz, then the 16-bit lo parts in FP32C), so one buffer descriptor
-// covers a whole level.
+// covers a whole level; level l starts l * PF_LEVEL_STAGGER bytes into its
+// allocation (lvlbase[l] is the allocation, lvl[l][0] the level).
 struct pf_ctx {
     int nx = 0, ny = 0, nz = 0, pz = 0;
     long long plane = 0, cells = 0;  // ny*pz, nx*ny*pz
