@@ -320,11 +320,12 @@ int pf::ctx_init(pf_ctx* c, const pf_desc* d) {
     // Level l starts l * PF_LEVEL_STAGGER bytes into its allocation.  The
     // allocations' bases are aligned alike, so the same node of every level
     // would sit at the same low address bits, and a sweep reads and writes
-    // the levels at neighbouring nodes together; an odd multiple of 256 B
-    // puts them on different memory channels (measured on the 513x513x257-
-    // node crystal: the two-step pass 1.3-1.5 % faster with 256, 2304, 4352
-    // or 8448 B, no gain with 1024, 4096 or 4608; the FP32C one-step sweep
-    // 0.4 %, FP64 unchanged).
+    // the levels at neighbouring nodes together: their lines compete for the
+    // same cache sets.  An odd multiple of 256 B spreads them (measured on
+    // the 513x513x257-node crystal: the two-step pass, which streams four
+    // levels, moved 7.6 % fewer bytes and ran 1.3-1.5 % faster with 256,
+    // 2304, 4352 or 8448 B, not with 1024, 4096 or 4608; the one-step
+    // sweeps, two levels, changed by 0.2-0.4 %).
     const size_t bytes = (size_t)c->cells * c->esz;
     const size_t lbytes = (size_t)c->cells * sizeof(lo_t);
     const size_t lvl_bytes = level_bytes(c->cells, c->precision);
