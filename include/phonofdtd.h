@@ -209,6 +209,8 @@ int pf_energy(pf_ctx *ctx, double *out);
  *   out[b, k] = scale * sum_t (in[b, t] * window[t]) exp(-2 pi i k t / n),
  * in (batch, n) row-major, out (batch, nf).  in_im and window may be NULL
  * (real input, no window).  Host arrays; runs on the current device.
+ * O(n log n) per trace: radix-2 FFT for a power of two n <= 8192, the
+ * chirp-z (Bluestein) form otherwise; n <= 2^23, batch <= 65535.
  */
 int pf_dft(const double *in_re, const double *in_im, int64_t batch, int64_t n, int64_t nf,
            const double *window, double scale, double *out_re, double *out_im);

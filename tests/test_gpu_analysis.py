@@ -1,7 +1,13 @@
 """Spectral analysis on the GPU (pf_dft) against numpy's FFT, which is what
-the reference's analysis.spectrum calls (analysis.py:44-96).  The direct
-FP64 DFT and numpy's FFT round differently, so the bar is 1e-11 of the
-largest magnitude of each transform."""
+the reference's analysis.spectrum calls (analysis.py:44-96).  pf_dft's FP64
+FFT (radix-2, Bluestein's chirp form for lengths that are not powers of two)
+and numpy's FFT round differently, so the bar is 1e-11 of the largest
+magnitude of each transform.
+
+Lengths by path: a power of two up to 8192 is a plain FFT in one block;
+any other length whose chirp convolution (M >= 2n - 1) fits 8192 values
+runs the chirp form in one block (3 ... 4096); longer traces, and powers of
+two above 8192, run the three-launch four-step chirp form."""
 import numpy as np
 import pytest
 
@@ -19,7 +25,7 @@ def _close(a, b):
     assert float(np.max(np.abs(a - b))) <= TOL * scale
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 7, 64, 100, 1000, 1001, 4096])
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 64, 100, 1000, 1001, 4095, 4096, 8192, 4097, 10007, 16384, 100003])
 def test_dft_matches_numpy(n):
     rng = np.random.default_rng(n)
     x = rng.standard_normal((5, n))
@@ -36,6 +42,31 @@ def test_dft_of_a_sinusoid_peaks_at_its_bin():
     y = np.abs(fdtd.dft(np.cos(2 * np.pi * k * t / n)))
     assert int(np.argmax(y[: n // 2])) == k
     assert abs(y[k] - n / 2) < 1e-9 * n
+
+
+def test_long_probe_traces():
+    # many long traces in one call (four-step path, several scratch rows)
+    rng = np.random.default_rng(3)
+    n = 50000
+    x = rng.standard_normal((48, n))
+    w = np.hanning(n)
+    _close(fdtd.dft(x, nf=n // 2, window=w, scale=1 / np.sqrt(n)),
+           (np.fft.fft(x * w, norm="ortho"))[:, : n // 2])
+
+
+def test_dft_is_linear_and_shifts_to_a_phase():
+    # X of x(t - s) (circular) is X_k e^{-2 pi i k s / n}: checks the chirp
+    # form's index arithmetic independently of numpy, at a four-step length
+    rng = np.random.default_rng(4)
+    n, sft = 30011, 977
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    y = rng.standard_normal(n)
+    X, Y, S = fdtd.dft(x), fdtd.dft(y), fdtd.dft(np.roll(x, sft))
+    k = np.arange(n)
+    _close(S, X * np.exp(-2j * np.pi * ((k * sft) % n) / n))
+    _close(fdtd.dft(2.5 * x - y), 2.5 * X - Y)
+    # Parseval
+    assert abs(np.sum(np.abs(X) ** 2) / n - np.sum(np.abs(x) ** 2)) < 1e-10 * np.sum(np.abs(x) ** 2)
 
 
 def test_many_traces_in_one_call():

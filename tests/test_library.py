@@ -105,3 +105,15 @@ def test_context_wrapper_raises_fdtd_error():
     with pytest.raises(fdtd.FdtdError, match="PF_ERR_ARG"):
         fdtd.Context(spec)
     del x
+
+
+def test_dft_arguments_are_checked_before_any_device_work():
+    # pf_dft's longest transform is 2^23 values (the chirp convolution of
+    # length 2^24); argument errors return before the device is touched
+    with pytest.raises(fdtd.FdtdError, match="longest transform"):
+        fdtd.dft(np.zeros(2 ** 23 + 1), nf=1)
+    lib = fdtd.load()
+    out = np.zeros(4)
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    assert lib.pf_dft(vp(out), None, 1, 4, 5, None, 1.0, vp(out), vp(out)) == -1
+    assert "nf" in lib.pf_last_error().decode()
