@@ -130,10 +130,36 @@ int check_desc(const pf_desc* d) {
 }  // namespace
 
 // --------------------------------------------------------- life cycle ----
+// Whether every spacing is a power of two whose reciprocal is a normal
+// double: then 1/h is exact and a product with it is bit-identical to the
+// division by h.
+static bool pow2_spacings(const pf_desc* d) {
+    const double* t[6] = {d->fdx, d->fdy, d->fdz, d->sdx, d->sdy, d->sdz};
+    const int n[6] = {d->nx - 1, d->ny - 1, d->nz - 1, d->nx - 2, d->ny - 2, d->nz - 2};
+    for (int a = 0; a < 6; a++)
+        for (int i = 0; i < n[a]; i++) {
+            int e = 0;
+            const double f = std::frexp(t[a][i], &e);
+            if (!std::isfinite(t[a][i]) || f != 0.5 || e < -1000 || e > 1000) return false;
+        }
+    return true;
+}
+
 template <class R>
 static int create_tables(pf_ctx* c, const pf_desc* d) {
     const bool exact = d->precision == PF_FP64;
     int rc;
+    // the FP64 one-step tiled sweep on power-of-two spacings multiplies by
+    // exact reciprocals (MODE_FP64P)
+    c->recip = exact && c->kernel == PF_KERNEL_TILED && pow2_spacings(d);
+    if (c->recip) {
+        if ((rc = upload_table(&c->rfx, spacing_table<double>(d->fdx, d->nx - 1, true)))) return rc;
+        if ((rc = upload_table(&c->rfy, spacing_table<double>(d->fdy, d->ny - 1, true)))) return rc;
+        if ((rc = upload_table(&c->rfz, spacing_table<double>(d->fdz, d->nz - 1, true)))) return rc;
+        if ((rc = upload_table(&c->rsx, spacing_table<double>(d->sdx, d->nx - 2, true)))) return rc;
+        if ((rc = upload_table(&c->rsy, spacing_table<double>(d->sdy, d->ny - 2, true)))) return rc;
+        if ((rc = upload_table(&c->rsz, spacing_table<double>(d->sdz, d->nz - 2, true)))) return rc;
+    }
     if ((rc = upload_table(&c->fx, spacing_table<R>(d->fdx, d->nx - 1, !exact)))) return rc;
     if ((rc = upload_table(&c->fy, spacing_table<R>(d->fdy, d->ny - 1, !exact)))) return rc;
     if ((rc = upload_table(&c->fz, spacing_table<R>(d->fdz, d->nz - 1, !exact)))) return rc;
@@ -422,6 +448,7 @@ void pf::ctx_free(pf_ctx* c) {
         F(c->lvlbase[l]);
     F(c->ids);
     F(c->mat); F(c->fx); F(c->fy); F(c->fz); F(c->sx); F(c->sy); F(c->sz);
+    F(c->rfx); F(c->rfy); F(c->rfz); F(c->rsx); F(c->rsy); F(c->rsz);
     F(c->dpx); F(c->dpy); F(c->dpz);
     F(c->mat64); F(c->rho64); F(c->fx64); F(c->fy64); F(c->fz64); F(c->sx64); F(c->sy64); F(c->sz64);
     F(c->wave); F(c->d_step); F(c->ring); F(c->d_frame); F(c->e_out); F(c->e_partials);
@@ -491,14 +518,19 @@ template Mesh<double> pf::make_view<double>(const pf_ctx*, int, int, int);
 // Launch the sweep of one step of mesh view m on stream s.
 template <class R, int MODE>
 static int launch_sweep_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s) {
-    if (c->kernel == PF_KERNEL_TILED || c->kernel == PF_KERNEL_TILED2) return launch_tiled<R, MODE>(c, m, s);
-    if (c->kernel == PF_KERNEL_STRIP) return launch_strip<R, MODE>(c, m, s);
-    dim3 blk(64, 4, 1);
-    dim3 grd((unsigned)((c->nz + 63) / 64), (unsigned)((c->ny + 3) / 4), (unsigned)c->nx);
-    if (c->sponge) hipLaunchKernelGGL((k_sweep_naive<R, MODE, true>), grd, blk, 0, s, m);
-    else hipLaunchKernelGGL((k_sweep_naive<R, MODE, false>), grd, blk, 0, s, m);
-    HIPCHK(hipGetLastError());
-    return PF_OK;
+    if constexpr (MODE == MODE_FP64P) {  // built for the one-step tiled sweep only
+        if (c->kernel != PF_KERNEL_TILED) return fail(PF_ERR_STATE, "reciprocal spacings need the tiled sweep");
+        return launch_tiled<R, MODE>(c, m, s);
+    } else {
+        if (c->kernel == PF_KERNEL_TILED || c->kernel == PF_KERNEL_TILED2) return launch_tiled<R, MODE>(c, m, s);
+        if (c->kernel == PF_KERNEL_STRIP) return launch_strip<R, MODE>(c, m, s);
+        dim3 blk(64, 4, 1);
+        dim3 grd((unsigned)((c->nz + 63) / 64), (unsigned)((c->ny + 3) / 4), (unsigned)c->nx);
+        if (c->sponge) hipLaunchKernelGGL((k_sweep_naive<R, MODE, true>), grd, blk, 0, s, m);
+        else hipLaunchKernelGGL((k_sweep_naive<R, MODE, false>), grd, blk, 0, s, m);
+        HIPCHK(hipGetLastError());
+        return PF_OK;
+    }
 }
 
 template <class R>
@@ -551,17 +583,32 @@ static int launch_faces_probe_t(const pf_ctx* c, const Mesh<R>& m, hipStream_t s
     return launch_probe_t<R>(c, m, s);
 }
 
+// The view a sweep of MODE reads: with MODE_FP64P the spacing tables and
+// first spacings are the exact reciprocals.
+template <class R, int MODE>
+static Mesh<R> sweep_view(const pf_ctx* c, int cur, int old, int nxt) {
+    Mesh<R> m = make_view<R>(c, cur, old, nxt);
+    if constexpr (MODE == MODE_FP64P) {
+        m.fx = (const R*)c->rfx; m.fy = (const R*)c->rfy; m.fz = (const R*)c->rfz;
+        m.sx = (const R*)c->rsx; m.sy = (const R*)c->rsy; m.sz = (const R*)c->rsz;
+        m.fx0 = (R)(1.0 / c->f0[0]); m.fy0 = (R)(1.0 / c->f0[1]); m.fz0 = (R)(1.0 / c->f0[2]);
+        m.sx0 = (R)(1.0 / c->s0[0]); m.sy0 = (R)(1.0 / c->s0[1]); m.sz0 = (R)(1.0 / c->s0[2]);
+    }
+    return m;
+}
+
 template <class R, int MODE>
 static int step_once_t(pf_ctx* c, hipStream_t s, const long long* base, long long off,
                        hipEvent_t* ev /* 2 or null */) {
-    Mesh<R> m = make_view<R>(c, c->cur, c->old, c->old);
+    Mesh<R> m = sweep_view<R, MODE>(c, c->cur, c->old, c->old);
     m.step_base = base;
     m.step_off = off;
     int rc;
     if (ev) HIPCHK(hipEventRecord(ev[0], s));
     if ((rc = launch_sweep_t<R, MODE>(c, m, s))) return rc;
     if (ev) HIPCHK(hipEventRecord(ev[1], s));
-    if ((rc = launch_faces_probe_t<R, MODE>(c, m, s))) return rc;
+    // the faces read no spacings: the FP64 face kernels serve MODE_FP64P
+    if ((rc = launch_faces_probe_t<R, MODE == MODE_FP64P ? MODE_FP64 : MODE>(c, m, s))) return rc;
     const int t = c->cur;
     c->cur = c->old;
     c->old = t;
@@ -583,7 +630,7 @@ template <class R, int MODE>
 static int tune_shape_t(pf_ctx* c) {
     static const int f32[][2] = {{16, 64}, {8, 64}, {32, 32}, {16, 32}, {8, 32}};
     static const int f64[][2] = {{8, 32}, {16, 32}, {8, 64}};
-    const bool d64 = MODE == MODE_FP64;
+    const bool d64 = MODE == MODE_FP64 || MODE == MODE_FP64P;
     const int (*cand)[2] = d64 ? f64 : f32;
     const int nc = d64 ? 3 : 5;
     if (c->wave_len < 1) return PF_OK;  // the sweep would return at once
@@ -594,7 +641,7 @@ static int tune_shape_t(pf_ctx* c) {
         (void)hipGetLastError();
         return fail(PF_ERR_HIP, "tile tuning: cannot create events");
     }
-    const Mesh<R> m = make_view<R>(c, c->cur, c->old, c->old);
+    const Mesh<R> m = sweep_view<R, MODE>(c, c->cur, c->old, c->old);
     const int j0 = c->tile_j, z0 = c->tile_z;
     int rc = PF_OK;
     float best[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
@@ -636,7 +683,7 @@ static int tune_shape_t(pf_ctx* c) {
 
 static int tune_shape(pf_ctx* c) {
     switch (c->precision) {
-        case PF_FP64: return tune_shape_t<double, MODE_FP64>(c);
+        case PF_FP64: return c->recip ? tune_shape_t<double, MODE_FP64P>(c) : tune_shape_t<double, MODE_FP64>(c);
         case PF_FP32: return tune_shape_t<float, MODE_FP32>(c);
         default: return tune_shape_t<float, MODE_FP32C>(c);
     }
@@ -707,7 +754,9 @@ int pf::step_pair(pf_ctx* c, hipStream_t s, const long long* base, long long off
 
 int pf::step_once(pf_ctx* c, hipStream_t s, const long long* base, long long off, hipEvent_t* ev) {
     switch (c->precision) {
-        case PF_FP64: return step_once_t<double, MODE_FP64>(c, s, base, off, ev);
+        case PF_FP64:
+            return c->recip ? step_once_t<double, MODE_FP64P>(c, s, base, off, ev)
+                            : step_once_t<double, MODE_FP64>(c, s, base, off, ev);
         case PF_FP32: return step_once_t<float, MODE_FP32>(c, s, base, off, ev);
         default: return step_once_t<float, MODE_FP32C>(c, s, base, off, ev);
     }

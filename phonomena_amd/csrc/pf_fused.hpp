@@ -94,7 +94,7 @@ PF_DEV Rsrc level_rsrc(const Mesh<R>& m, R* base) {
 template <class R, int MODE, bool SPONGE, int TJ, int KZ, bool FULL>
 PF_DEV void fused_march(const Mesh<R>& m, R* out2, FusedLds<R, TJ, KZ>& L, int kl, int row, int k, int j, int ib,
                         int ie, R w, R w1) {
-    constexpr bool EXACT = MODE == MODE_FP64;
+    constexpr int EXACT = exact_of(MODE);
     constexpr bool COMP = MODE == MODE_FP32C;
     constexpr int RB = MAT_STRIDE * (int)sizeof(R);
     const int rp = row + 1 < TJ ? row + 1 : TJ - 1;

@@ -111,7 +111,7 @@ template <class R> PF_DEV const R* matp(const Mesh<R>& m, int i, int j, int k) {
 // ------------------------------------------------------------- stresses ----
 // Each returns the value the reference's T array holds at (i,j,k) after
 // update_T + apply_T_tfbc, and 0 where the reference never writes it.
-template <bool EXACT, class R>
+template <int EXACT, class R>
 PF_DEV void tn(const Mesh<R>& m, int i, int j, int k, R w, R& t1, R& t2, R& t3) {
     t1 = t2 = t3 = R(0);
     if (i < 1 || i > m.nx - 2 || j < 1 || j > m.ny - 2 || k < 0 || k > m.nz - 2) return;
@@ -126,7 +126,7 @@ PF_DEV void tn(const Mesh<R>& m, int i, int j, int k, R w, R& t1, R& t2, R& t3) 
     }
 }
 
-template <bool EXACT, class R>
+template <int EXACT, class R>
 PF_DEV R t4(const Mesh<R>& m, int i, int j, int k, R w) {
     if (i < 1 || i > m.nx - 2 || j < 0 || j > m.ny - 2 || k < 0 || k > m.nz - 2) return R(0);
     if (k >= 1) {
@@ -139,7 +139,7 @@ PF_DEV R t4(const Mesh<R>& m, int i, int j, int k, R w) {
                         luz(m, i, j + 1, 0, w) - luz(m, i, j, 0, w), m.fz0);
 }
 
-template <bool EXACT, class R>
+template <int EXACT, class R>
 PF_DEV R t5(const Mesh<R>& m, int i, int j, int k, R w) {
     if (i < 0 || i > m.nx - 2 || j < 1 || j > m.ny - 2 || k < 0 || k > m.nz - 2) return R(0);
     if (k >= 1) {
@@ -152,7 +152,7 @@ PF_DEV R t5(const Mesh<R>& m, int i, int j, int k, R w) {
                         luz(m, i + 1, j, 0, w) - luz(m, i, j, 0, w), m.fz0);
 }
 
-template <bool EXACT, class R>
+template <int EXACT, class R>
 PF_DEV R t6(const Mesh<R>& m, int i, int j, int k, R w) {
     if (i < 0 || i > m.nx - 2 || j < 0 || j > m.ny - 2 || k < 0 || k > m.nz - 2) return R(0);
     (void)w;
@@ -173,7 +173,7 @@ PF_DEV R damp_at(const Mesh<R>& m, int i, int j, int k) {
 
 // Stage-I bracket ("div") and coefficient of each component at a non-face
 // cell (base_solver.py:441-463 for k >= 1, :495-517 for k = 0).
-template <bool EXACT, class R>
+template <int EXACT, class R>
 PF_DEV void stage_ux(const Mesh<R>& m, int i, int j, int k, R w, R& coef, R& dv) {
     R a1, a2, a3, b1, b2, b3;
     tn<EXACT>(m, i + 1, j, k, w, a1, a2, a3);
@@ -190,7 +190,7 @@ PF_DEV void stage_ux(const Mesh<R>& m, int i, int j, int k, R w, R& coef, R& dv)
     }
 }
 
-template <bool EXACT, class R>
+template <int EXACT, class R>
 PF_DEV void stage_uy(const Mesh<R>& m, int i, int j, int k, R w, R& coef, R& dv) {
     R d6 = t6<EXACT>(m, i, j, k, w) - t6<EXACT>(m, i - 1, j, k, w);
     R a1, a2, a3, b1, b2, b3;
@@ -207,7 +207,7 @@ PF_DEV void stage_uy(const Mesh<R>& m, int i, int j, int k, R w, R& coef, R& dv)
     }
 }
 
-template <bool EXACT, class R>
+template <int EXACT, class R>
 PF_DEV void stage_uz(const Mesh<R>& m, int i, int j, int k, R w, R& coef, R& dv) {
     R d5 = t5<EXACT>(m, i, j, k, w) - t5<EXACT>(m, i - 1, j, k, w);
     R d4 = t4<EXACT>(m, i, j, k, w) - t4<EXACT>(m, i, j - 1, k, w);
@@ -598,7 +598,7 @@ PF_DEV V<R> leap_from(V<R> cur, R oh, R ol, R coef, R dv, R s) {
     if constexpr (MODE == MODE_FP32C) {
         leap_comp<SPONGE>(cur.h, cur.l, oh, ol, coef * dv, s, r.h, r.l);
     } else {
-        r.h = leap<MODE == MODE_FP64, SPONGE>(cur.h, oh, coef, dv, s);
+        r.h = leap<exact_of(MODE), SPONGE>(cur.h, oh, coef, dv, s);
         r.l = R(0);
     }
     return r;
@@ -616,7 +616,7 @@ PF_DEV V<R> leap_value(V<R> cur, const R* nh, const lo_t* nl, long long id, R co
 // above them.
 template <class R, int MODE, bool SPONGE>
 PF_DEV void naive_cell(const Mesh<R>& m, int i, int j, int k, R w) {
-    constexpr bool EXACT = MODE == MODE_FP64;
+    constexpr int EXACT = exact_of(MODE);
     const long long id = at(m, i, j, k);
     const R s = SPONGE ? damp_at(m, i, j, k) : R(0);
     // ux: non-face cells i <= nx-3, 1 <= j <= ny-2, k <= nz-2

@@ -55,6 +55,11 @@ struct pf_ctx {
     void *mat = nullptr, *fx = nullptr, *fy = nullptr, *fz = nullptr, *sx = nullptr, *sy = nullptr,
          *sz = nullptr;
     void *dpx = nullptr, *dpy = nullptr, *dpz = nullptr;
+    // FP64 with every spacing a power of two: the reciprocal tables of the
+    // one-step tiled sweep's products (MODE_FP64P); fx.. keep the spacings
+    // for the other kernels (batch, energy).
+    bool recip = false;
+    void *rfx = nullptr, *rfy = nullptr, *rfz = nullptr, *rsx = nullptr, *rsy = nullptr, *rsz = nullptr;
     void *mat64 = nullptr, *rho64 = nullptr, *fx64 = nullptr, *fy64 = nullptr, *fz64 = nullptr,
          *sx64 = nullptr, *sy64 = nullptr, *sz64 = nullptr;
     double f0[3] = {}, s0[3] = {};

@@ -31,7 +31,7 @@ struct StripMap {
 
 template <class R, int MODE, bool SPONGE, int NR, int W>
 __global__ __launch_bounds__(64 * W, (MODE == MODE_FP32 ? 4 : 2)) void k_sweep_strip(Mesh<R> m, StripMap sm) {
-    constexpr bool EXACT = MODE == MODE_FP64;
+    constexpr int EXACT = exact_of(MODE);
     constexpr bool COMP = MODE == MODE_FP32C;
     constexpr int NRR = NR + 2;
     __shared__ R smat[PF_TILED_MAX_MATERIALS * MAT_STRIDE];

@@ -283,7 +283,7 @@ PF_DEV double above0(double v) {
 template <class R, int MODE, bool SPONGE, int TJ, int KZ, bool FYZ, bool FX>
 PF_DEV void tiled_march(const Mesh<R>& m, TiledLds<R, TJ, KZ>& L, int kl, int row, int k, int j, int ib, int ie,
                         R w) {
-    constexpr bool EXACT = MODE == MODE_FP64;
+    constexpr int EXACT = exact_of(MODE);
     constexpr bool COMP = MODE == MODE_FP32C;
     constexpr int RB = MAT_STRIDE * (int)sizeof(R);  // bytes per material row
     const int rp = row + 1 < TJ ? row + 1 : TJ - 1;

@@ -78,6 +78,28 @@ def test_fp64_crystal_two_materials():
     ctx.close()
 
 
+@pytest.mark.parametrize("spacing", [(0.5, 2.0, 0.25), (1.0, 1.0, 1.0), (0.3, 0.7, 0.3)])
+@pytest.mark.parametrize("sponge", [False, True])
+def test_fp64_spacings_bitwise_vs_oracle(spacing, sponge):
+    """On power-of-two spacings the FP64 tiled sweep takes each division by
+    a spacing as a product with its exact reciprocal; on any other spacing it
+    divides.  Both must give the oracle's bits (which divides), on every
+    tile shape and with or without the absorbing layer."""
+    cells = (36, 30, 20)
+    _, _, _, ids, rows, dt, wave = uniform_crystal(
+        cells, min(spacing), incl=(19300.0, 2.0e11, 1.6e11, 12, 4), steps=160, seed=5)
+    x, y, z = (np.arange(n + 1) * h for n, h in zip(cells, spacing))
+    prof = problem.sponge_tables((x.size, y.size, z.size), {"width": 4, "strength": 0.2}) if sponge else None
+    st, _ = _oracle_brute(x, y, z, ids, rows, dt, wave, 160, sponge=prof)
+    for tj, kz in ((8, 32), (16, 64), (32, 32)):
+        ctx = fdtd.Context(spec_from(x, y, z, ids, rows, dt, wave, sponge=prof), precision="fp64",
+                           kernel="tiled", tile_j=tj, tile_z=kz)
+        ctx.step(160)
+        for f in ("ux", "uy", "uz"):
+            assert np.array_equal(ctx.get(f), st[f]), (tj, kz, f, rel_l2(ctx.get(f), st[f]))
+        ctx.close()
+
+
 @pytest.mark.parametrize("precision,tol", [("fp32", 1e-4), ("fp32c", 1e-5)])
 def test_fp32_modes_vs_fp64_oracle(precision, tol):
     x, y, z, ids, rows, dt, wave = uniform_crystal(
