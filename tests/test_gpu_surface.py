@@ -32,9 +32,10 @@ def _box_spec(h, steps):
     return spec
 
 
-@pytest.mark.parametrize("kernel", ["tiled", "naive", "strip"])
-def test_fp64_fixed_surface_equals_oracle_at_physical_spacing(kernel):
-    steps, h = 60, 1e-4
+@pytest.mark.parametrize("kernel,h", [("tiled", 1e-4), ("naive", 1e-4), ("strip", 1e-4), ("tiled", 2.0 ** -13)])
+def test_fp64_fixed_surface_equals_oracle_at_physical_spacing(kernel, h):
+    # h = 2^-13 m (1.22e-4): the tiled sweep's reciprocal-product flavour
+    steps = 60
     x, y, z, ids, cs, rhos, dt, wave = box_inputs(h, steps)
     C, P = rm.dense_material(ids, cs, rhos)
     ref, _ = rm.run(rm.Problem(x, y, z, C, P, dt, wave, surface_fixed=True), steps)
