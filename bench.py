@@ -374,22 +374,29 @@ def bench_physical(steps=1000):
             "fp32_vs_fp64": rel(run(phys, "fp32")), "fp32c_vs_fp64": rel(run(phys, "fp32c"))}
 
 
-def bench_served(n_jobs, capacity_jobs_per_s, precision="fp32c", load=0.7, seed=5, chunk=200):
+def bench_served(n_jobs, capacity_jobs_per_s, precision="fp32c", load=0.7, seed=5, chunk=200, hbm_budget=200e9):
     """Served design-sweep stream: Poisson arrivals at `load` x the batched
     capacity (jobs/s), grids U[32^3 .. 256x256x128] cells, U{500..8000}
-    steps, lattice parameters as in the batched mix, 200 GB HBM budget,
-    32x32 probe frames streamed back every 10 steps."""
+    steps, lattice parameters as in the batched mix, an HBM budget for the
+    resident jobs (200 GB: never binding for this stream; a small one makes
+    arrivals wait for memory), 32x32 probe frames streamed back every 10
+    steps."""
     from phonomena_amd import serve
     jobs = serve.stream_jobs(n_jobs, seed)
     rate = load * capacity_jobs_per_s
     arrivals = serve.poisson_arrivals(n_jobs, rate, seed)
-    srv = serve.Server(precision=precision, hbm_budget=200e9, chunk=chunk, probe={"count": 32}, record_every=10)
+    srv = serve.Server(precision=precision, hbm_budget=hbm_budget, chunk=chunk, probe={"count": 32},
+                       record_every=10)
     results, stats = srv.run(jobs, arrivals)
     stats.update({"workload": "served stream of %d jobs, seed %d: Poisson arrivals at %.2f jobs/s (%.1f x the "
                               "batched capacity), 32^3..256x256x128-cell crystals, 500-8000 steps, %s, "
-                              "200 GB HBM budget, 32x32 probes every 10 steps streamed back"
-                              % (n_jobs, seed, rate, load, precision),
+                              "%.3g GB HBM budget, 32x32 probes every 10 steps streamed back"
+                              % (n_jobs, seed, rate, load, precision, hbm_budget / 1e9),
                   "arrival_rate_jobs_per_s": rate})
+    waits = np.array([r.admitted - r.arrival for r in results if r is not None and r.admitted is not None])
+    if waits.size:  # arrival to admission (memory and chunk boundaries)
+        stats["admission_wait_s"] = {"mean": float(waits.mean()), "p99": float(np.percentile(waits, 99)),
+                                     "max": float(waits.max())}
     return stats
 
 
@@ -439,6 +446,10 @@ def main():
             jobs = serve.stream_jobs(256, 5)
             mean_work = np.mean([np.prod([c + 1 for c in j["cells"]]) * j["steps"] for j in jobs])
             rec["served"] = bench_served(args.serve_jobs, rec["batched"]["value"] / mean_work)
+            # the same stream with a budget of about half its resident peak:
+            # admission by memory, arrivals queue for it
+            rec["served_budget_bound"] = bench_served(args.serve_jobs, rec["batched"]["value"] / mean_work,
+                                                      hbm_budget=0.5e9)
     if args.precision_steps > 0:
         rec["precision_case"] = bench_precision(args.precision_steps)
     if not args.no_setup:
