@@ -178,7 +178,10 @@ def bench_plain(args):
                                                            else "16-cell absorbing layer")),
                    "nodes": nodes, "kernel": kernel, "kernel_requested": args.kernel, "tile_j": tile_j,
                    "tile_z": tile_z, "tile_x": args.tile_x,
-                   "fill_fraction": float(m.ids.mean())},
+                   "fill_fraction": float(m.ids.mean()),
+                   # every spacing a power of two: the FP64 sweep multiplies by
+                   # exact reciprocals instead of dividing (same bits)
+                   "spacings_power_of_two": bool(all(np.all(np.frexp(a)[0] == 0.5) for a in spec.fd + spec.sd))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      # the same algorithmic bytes over the whole timed step
@@ -206,6 +209,7 @@ def bench_other_precisions(args, precisions=("fp32c", "fp64")):
         out[prec] = {"value": r["value"], "unit": r["unit"], "ms_per_step": r["ms_per_step"], "dtype": r["dtype"],
                      "kernel": r["config"]["kernel"], "tile_j": r["config"]["tile_j"],
                      "tile_z": r["config"]["tile_z"],
+                     "spacings_power_of_two": r["config"]["spacings_power_of_two"],
                      "roofline": {k: r["roofline"][k] for k in ("achieved", "peak", "unit", "frac", "frac_whole_step",
                                                                  "bytes_per_node", "kernel_bytes_per_node",
                                                                  "sweep_ms", "face_ms", "traffic")}}
