@@ -54,6 +54,15 @@ def test_long_probe_traces():
            (np.fft.fft(x * w, norm="ortho"))[:, : n // 2])
 
 
+@pytest.mark.parametrize("n", [2 ** 23, 2 ** 23 - 1])
+def test_longest_transforms(n):
+    # the largest chirp convolution (2^24 values): one column and one row
+    # per four-step block, 256 MB of scratch per trace
+    rng = np.random.default_rng(n % 1000)
+    x = rng.standard_normal((2, n))
+    _close(fdtd.dft(x, nf=4096, scale=0.5), np.fft.fft(x)[:, :4096] * 0.5)
+
+
 def test_dft_is_linear_and_shifts_to_a_phase():
     # X of x(t - s) (circular) is X_k e^{-2 pi i k s / n}: checks the chirp
     # form's index arithmetic independently of numpy, at a four-step length
