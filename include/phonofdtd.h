@@ -32,7 +32,10 @@ extern "C" {
 #define PF_ERR_RANGE (-5)   /* step count beyond the uploaded source     */
 
 /* Arithmetic of the sweep. */
-#define PF_FP64 0   /* FP64, the reference's expression order (parity mode)   */
+#define PF_FP64 0   /* FP64, the reference's expression order (parity mode);
+                       when every spacing is a power of two the tiled sweep
+                       multiplies by the exact reciprocals instead of
+                       dividing, which gives the same bits                   */
 #define PF_FP32 1   /* FP32 fields, pre-folded coefficients (production)      */
 #define PF_FP32C 2  /* FP32 fields carried as hi+lo pairs: compensated
                        (TwoSum/Kahan) time integration for long runs         */
