@@ -436,3 +436,24 @@ def test_material_table_rejects_mismatched_arrays():
     m.P = np.ones((4, 4, 4))
     with pytest.raises(ValueError, match="nodes"):
         problem.material_table(m, (5, 4, 4))
+
+
+def test_power_of_two_reciprocal_products_equal_divisions():
+    # The FP64 sweep on power-of-two spacings takes x / h as x * (1/h): for
+    # h = 2^k both are the correctly rounded value of the same real number,
+    # including subnormal, overflowing and signed-zero results; for other
+    # spacings the two differ on some operands (a few special divisors, such
+    # as 0.3, happen to agree on every operand tried: only powers of two are
+    # used).
+    rng = np.random.default_rng(11)
+    x = np.concatenate([rng.standard_normal(20000) * 10.0 ** rng.integers(-300, 300, 20000),
+                        rng.standard_normal(2000) * 1e-310,          # subnormal
+                        np.array([0.0, -0.0, 5e-324, -5e-324, 1.7e308, -1.7e308, np.inf, -np.inf])])
+    with np.errstate(over="ignore", under="ignore"):
+        for k in range(-40, 41):
+            h = np.float64(2.0) ** k
+            r = np.float64(1.0) / h
+            assert (x * r).view(np.int64).tolist() == (x / h).view(np.int64).tolist(), k
+        # the reference's physical spacing and another non-power of two
+        for h in (np.float64(1e-4), np.float64(0.7)):
+            assert not np.array_equal((x * (1.0 / h)).view(np.int64), (x / h).view(np.int64)), h
