@@ -295,14 +295,15 @@ extern "C" int pf_dft(const double* in_re, const double* in_im, int64_t batch, i
     if (!in_re || !out_re || !out_im) return fail(PF_ERR_ARG, "NULL argument");
     if (batch < 1 || n < 1 || nf < 1 || nf > n) return fail(PF_ERR_ARG, "need batch, n >= 1 and 1 <= nf <= n");
     if (batch > 65535) return fail(PF_ERR_ARG, "batch must be <= 65535 (got %lld)", (long long)batch);
+    // n <= 2^(FFT_LG_MAX-1): every length's chirp convolution fits 2^FFT_LG_MAX
+    if (n > (1ll << (FFT_LG_MAX - 1)))
+        return fail(PF_ERR_ARG, "n = %lld exceeds the longest transform (%lld)", (long long)n,
+                    (1ll << (FFT_LG_MAX - 1)));
     // a power of two that fits one block: plain FFT of length n; otherwise
     // the chirp form over M >= 2n - 1 (the four-step path always takes it)
     const int lgn = ceil_lg(n);
     const bool blue = (1ll << lgn) != n || lgn > FFT_LG_SMALL;
     const int lg = blue ? ceil_lg(2 * n - 1) : lgn;
-    if (lg > FFT_LG_MAX)
-        return fail(PF_ERR_ARG, "n = %lld exceeds the longest transform (%lld)", (long long)n,
-                    (1ll << (FFT_LG_MAX - 1)));
     const long long M = 1ll << lg;
     const bool small = lg <= FFT_LG_SMALL;
     const int lg1 = lg / 2, lg2 = lg - lg1;

@@ -113,6 +113,12 @@ def test_dft_arguments_are_checked_before_any_device_work():
     with pytest.raises(fdtd.FdtdError, match="longest transform"):
         fdtd.dft(np.zeros(2 ** 23 + 1), nf=1)
     lib = fdtd.load()
+    one = np.zeros(1)
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    # lengths far beyond the limit fail the same way (no shift or size overflow)
+    assert lib.pf_dft(vp(one), None, 1, 2 ** 62 + 1, 1, None, 1.0, vp(one), vp(one)) == -1
+    assert "longest transform" in lib.pf_last_error().decode()
+    lib = fdtd.load()
     out = np.zeros(4)
     vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
     assert lib.pf_dft(vp(out), None, 1, 4, 5, None, 1.0, vp(out), vp(out)) == -1
